@@ -1,0 +1,135 @@
+/*
+ * tpr_hip.h -- C ABI of the MI355X (gfx950) TPR/PTPR tridiagonal solver
+ * engine, libtpr_hip.so.
+ *
+ * Every system is  a[i] x[i-1] + b[i] x[i] + c[i] x[i+1] = d[i],  i = 0..n-1,
+ * stored as plain float arrays (structure of arrays, unit stride).  As in the
+ * reference, a[0] and c[n-1] are outside the matrix and are expected to be 0;
+ * n and the slice size s are powers of two with s <= n.
+ *
+ * Arithmetic modes:
+ *   TPRH_ARITH_EXACT  the reference CPU solver's operation order, IEEE
+ *                     division, no FMA: results are bit-identical to
+ *                     src/ptpr.cpp / src/tpr.cpp built for x86-64 (Release).
+ *   TPRH_ARITH_FAST   FMA and the hardware reciprocal in the per-equation
+ *                     loops; relative L2 distance to the EXACT result stays
+ *                     far below 1e-5 on diagonally dominant systems.
+ *
+ * Entry points and the reference interface each one replaces:
+ *   tprh_ptpr_cu / tprh_tpr_cu      PTPR_CU::ptpr_cu  src/ptpr.cuh:51, src/ptpr.cu:542-600
+ *                                   TPR_CU::tpr_cu    src/tpr.cuh:51,  src/tpr.cu:574-631
+ *   tprh_pcr_cu / tprh_cr_cu        PTPR_CU::pcr_cu   src/ptpr.cuh:52, src/ptpr.cu:683-707
+ *                                   TPR_CU::cr_cu     src/tpr.cuh:52,  src/tpr.cu:697-723
+ *   tprh_solver_*                   the solver classes PTPR(n,s) / TPR(n,s):
+ *                                   src/ptpr.hpp:53-97, src/tpr.hpp:53-97
+ *                                   (construct, set_tridiagonal_system 3- and
+ *                                   4-argument forms, solve, get_ans)
+ *   tprh_solve_batched              no reference counterpart: the reference
+ *                                   solves one system per launch; this solves
+ *                                   a device-resident batch in one launch
+ *   tprh_generate                   the seeded generator of the benchmark
+ *                                   inputs (the reference's own generators,
+ *                                   src/system.cpp:188-228, are unseeded)
+ *
+ * All functions return a tprh_status; none of them exits the process (the
+ * reference's CU_CHECK exits, src/ptpr.cu:509-517).  tprh_last_error() gives
+ * the message of the last failure on the calling thread.
+ */
+#ifndef TPR_HIP_H
+#define TPR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tprh_status {
+    TPRH_SUCCESS = 0,
+    TPRH_ERROR_INVALID_VALUE = 1, /* bad n, s, count or null pointer */
+    TPRH_ERROR_NOT_SUPPORTED = 2, /* valid, but outside what this build solves */
+    TPRH_ERROR_HIP = 3,           /* a HIP runtime call failed */
+    TPRH_ERROR_OUT_OF_MEMORY = 4,
+    TPRH_ERROR_BUDGET = 5         /* a serving request exceeds the memory budget */
+} tprh_status;
+
+typedef enum tprh_method {
+    TPRH_PTPR = 0, /* PCR-like tree partitioning reduction (src/ptpr.cpp) */
+    TPRH_TPR = 1,  /* tree partitioning reduction (src/tpr.cpp) */
+    TPRH_PCR = 2,  /* parallel cyclic reduction (src/pcr.cpp) */
+    TPRH_CR = 3    /* cyclic reduction (src/cr.cpp) */
+} tprh_method;
+
+typedef enum tprh_arith { TPRH_ARITH_FAST = 0, TPRH_ARITH_EXACT = 1 } tprh_arith;
+
+typedef enum tprh_gen {
+    TPRH_GEN_DOMINANT = 0,  /* a,c ~ U(-1,1), b = |a|+|c| + (1 + U[0,1)), xt ~ U(-1,1), d = A xt */
+    TPRH_GEN_WEAK = 1,      /* as DOMINANT but b = |a|+|c| + (1e-4 + U[0,1) * 0.0099) */
+    TPRH_GEN_FIXED = 2,     /* ExampleFixedInput: a = c = -1/6, b = 1, d = i + 1 */
+    TPRH_GEN_RANDOM_RHS = 3 /* ExampleRandomRHSInput: a = c = -1/6, b = 1, d ~ U(-1,1) */
+} tprh_gen;
+
+const char *tprh_status_string(tprh_status status);
+const char *tprh_last_error(void);
+int tprh_version(void);
+
+/* ---- drop-ins for the reference's CUDA free functions -------------------
+ * Host pointers, unit diagonal, synchronous, current HIP device.
+ * a, c, rhs are read only; x (n floats) receives the answer.
+ * Unlike the reference, n / s > s is solved (the reduced system is not
+ * limited to one block) and invalid (n, s) returns TPRH_ERROR_INVALID_VALUE
+ * instead of returning silently with x untouched. */
+tprh_status tprh_ptpr_cu(const float *a, const float *c, const float *rhs, float *x, int n, int s);
+tprh_status tprh_tpr_cu(const float *a, const float *c, const float *rhs, float *x, int n, int s);
+tprh_status tprh_pcr_cu(const float *a, const float *c, const float *rhs, float *x, int n);
+tprh_status tprh_cr_cu(const float *a, const float *c, const float *rhs, float *x, int n);
+
+/* ---- the solver-class surface --------------------------------------------
+ * tprh_solver_create(&h, TPRH_PTPR, n, s)      ~ PTPR t(n, s);
+ * tprh_solver_set_system(h, a, NULL, c, rhs)   ~ t.set_tridiagonal_system(a, c, rhs);
+ * tprh_solver_set_system(h, a, diag, c, rhs)   ~ t.set_tridiagonal_system(a, diag, c, rhs);
+ * tprh_solver_solve(h, &flops)                 ~ flops = t.solve();
+ * tprh_solver_get_ans(h, x)                    ~ t.get_ans(x);
+ * The solver keeps the host pointers given to set_system (as the reference
+ * does) and reads them in solve().  The division by diag happens on the GPU;
+ * the caller's arrays are not modified.  methods: TPRH_PTPR, TPRH_TPR (s used)
+ * and TPRH_PCR, TPRH_CR (s ignored). */
+typedef struct tprh_solver tprh_solver;
+tprh_status tprh_solver_create(tprh_solver **out, tprh_method method, int n, int s);
+tprh_status tprh_solver_set_arith(tprh_solver *h, tprh_arith arith); /* default EXACT */
+tprh_status tprh_solver_set_system(tprh_solver *h, const float *a, const float *diag, const float *c,
+                                   const float *rhs);
+tprh_status tprh_solver_solve(tprh_solver *h, int *flops);
+tprh_status tprh_solver_get_ans(tprh_solver *h, float *x);
+tprh_status tprh_solver_destroy(tprh_solver *h);
+
+/* ---- device-resident batches ----------------------------------------------
+ * nsys systems of n equations, back to back: system k occupies elements
+ * [k*n, (k+1)*n) of a, b, c, d and x, all device pointers.  b == NULL means a
+ * unit diagonal.  Asynchronous on `stream` (a hipStream_t; NULL = the null
+ * stream); no host synchronisation, no allocation, graph-capturable.
+ * method: TPRH_PTPR or TPRH_TPR. */
+tprh_status tprh_solve_batched(tprh_method method, tprh_arith arith, const float *a, const float *b,
+                               const float *c, const float *d, float *x, int n, int s, int nsys,
+                               void *stream);
+
+/* Same with an explicit slice shape: E equations per lane, L lanes per slice
+ * (E * L == s).  Used by the autotuner; tprh_shape_supported() lists what is
+ * compiled in. */
+tprh_status tprh_solve_batched_shape(tprh_method method, tprh_arith arith, const float *a, const float *b,
+                                     const float *c, const float *d, float *x, int n, int s, int nsys,
+                                     int E, int L, void *stream);
+int tprh_shape_supported(int E, int L);
+
+/* Synthetic systems with ids sys0 .. sys0+nsys-1 written back to back into
+ * device arrays a, b, c, d (and xt unless NULL); bit-identical to the
+ * oracle's generator on the host.  Asynchronous on `stream`. */
+tprh_status tprh_generate(tprh_gen kind, uint64_t seed, uint64_t sys0, int nsys, int n, float *a, float *b,
+                          float *c, float *d, float *xt, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPR_HIP_H */

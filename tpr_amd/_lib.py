@@ -1,0 +1,87 @@
+"""ctypes binding of libtpr_hip.so (the C ABI declared in include/tpr_hip.h).
+
+The library must have been built (``python -m tpr_amd.build``); there is no
+fallback: every entry point raises if the shared object cannot be loaded.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtpr_hip.so")
+
+SUCCESS = 0
+ERROR_INVALID_VALUE = 1
+ERROR_NOT_SUPPORTED = 2
+ERROR_HIP = 3
+ERROR_OUT_OF_MEMORY = 4
+ERROR_BUDGET = 5
+
+PTPR_M, TPR_M, PCR_M, CR_M = 0, 1, 2, 3
+ARITH_FAST, ARITH_EXACT = 0, 1
+GEN_DOMINANT, GEN_WEAK, GEN_FIXED, GEN_RANDOM_RHS = 0, 1, 2, 3
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+
+# name -> (restype, argtypes); one line per exported function of tpr_hip.h
+SIGNATURES = {
+    "tprh_status_string": (ctypes.c_char_p, [_i]),
+    "tprh_last_error": (ctypes.c_char_p, []),
+    "tprh_version": (_i, []),
+    "tprh_ptpr_cu": (_i, [_vp, _vp, _vp, _vp, _i, _i]),
+    "tprh_tpr_cu": (_i, [_vp, _vp, _vp, _vp, _i, _i]),
+    "tprh_pcr_cu": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "tprh_cr_cu": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "tprh_solver_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i]),
+    "tprh_solver_set_arith": (_i, [_vp, _i]),
+    "tprh_solver_set_system": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "tprh_solver_solve": (_i, [_vp, ctypes.POINTER(_i)]),
+    "tprh_solver_get_ans": (_i, [_vp, _vp]),
+    "tprh_solver_destroy": (_i, [_vp]),
+    "tprh_solve_batched": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "tprh_solve_batched_shape": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "tprh_shape_supported": (_i, [_i, _i]),
+    "tprh_generate": (_i, [_i, _u64, _u64, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class TprError(RuntimeError):
+    """A libtpr_hip call returned a status other than success."""
+
+    def __init__(self, status, func, msg):
+        self.status = status
+        super().__init__("%s failed: %s (%s)" % (func, msg, _status_name(status)))
+
+
+def _status_name(st):
+    return {0: "success", 1: "invalid value", 2: "not supported", 3: "HIP error",
+            4: "out of memory", 5: "budget exceeded"}.get(st, "status %d" % st)
+
+
+def lib():
+    """The loaded library; raises RuntimeError if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("tpr_amd/libtpr_hip.so is not built: run `python -m tpr_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, func):
+    if status != SUCCESS:
+        msg = lib().tprh_last_error()
+        raise TprError(status, func, msg.decode() if msg else "")
+    return status
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args), name)

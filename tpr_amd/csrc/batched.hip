@@ -1,0 +1,78 @@
+// batched.hip -- batched PTPR / TPR: many independent systems of the same
+// size, one group of lanes per system, one launch for the whole batch.
+//
+// This replaces the reference's one-system-per-launch cooperative kernels
+// (src/ptpr.cu:48-150, src/tpr.cu:44-156): a system lives inside one
+// workgroup, so there is no grid-wide barrier and no co-residency limit, and
+// the reduced system is solved by the system's own workgroup.
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+#include "slice_solvers.hpp"
+
+namespace tprh {
+
+template <int METHOD, int E, int L, int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void batched_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                                       const float* __restrict__ c, const float* __restrict__ d,
+                                                       float* __restrict__ x, int n, int m, int nsys,
+                                                       int sys_per_block) {
+    extern __shared__ float lds[];
+    const int T = m * L;  // lanes per system
+    const int sl = threadIdx.x / T;
+    const int t = threadIdx.x - sl * T;
+    const int sys = blockIdx.x * sys_per_block + sl;
+    const bool valid = sys < nsys;
+    const size_t base = valid ? (size_t)sys * (size_t)n : 0;
+    float* my = lds + sl * lds_words_per_system(m);
+    if constexpr (METHOD == 0) {
+        ptpr_solve_lanes<E, L, MODE>(a, b, c, d, x, base, m, t, valid, my);
+    } else {
+        tpr_solve_lanes<E, L, MODE>(a, b, c, d, x, base, m, t, valid, my);
+    }
+}
+
+namespace {
+
+template <int METHOD, int E, int L, int MODE>
+hipError_t launch_el(const BatchArgs& p, hipStream_t stream) {
+    const int T = p.m * L;
+    int block, spb;
+    if (T <= 64) {
+        block = 64;
+        spb = 64 / T;
+    } else {
+        block = T;
+        spb = 1;
+    }
+    const int grid = (p.nsys + spb - 1) / spb;
+    const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(p.m);
+    if (block == 64) {
+        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, 64>), dim3(grid), dim3(block), shmem, stream, p.a, p.b,
+                           p.c, p.d, p.x, p.n, p.m, p.nsys, spb);
+    } else {
+        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, 1024>), dim3(grid), dim3(block), shmem, stream, p.a,
+                           p.b, p.c, p.d, p.x, p.n, p.m, p.nsys, spb);
+    }
+    return hipGetLastError();
+}
+
+template <int METHOD, int MODE>
+hipError_t launch_mode(const BatchArgs& p, int E, int L, hipStream_t stream) {
+#define TPRH_EL(EE, LL) \
+    if (E == EE && L == LL) return launch_el<METHOD, EE, LL, MODE>(p, stream);
+    TPRH_SLICE_SHAPES(TPRH_EL)
+#undef TPRH_EL
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_batched(int method, int mode, const BatchArgs& p, int E, int L, hipStream_t stream) {
+    if (method == 0) {
+        return mode == kExact ? launch_mode<0, kExact>(p, E, L, stream) : launch_mode<0, kFast>(p, E, L, stream);
+    }
+    return mode == kExact ? launch_mode<1, kExact>(p, E, L, stream) : launch_mode<1, kFast>(p, E, L, stream);
+}
+
+}  // namespace tprh

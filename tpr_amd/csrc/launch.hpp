@@ -1,0 +1,68 @@
+// launch.hpp -- host-side interface between the C ABI (capi.hip) and the
+// kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tprh {
+
+// Slice shapes (E equations per lane, L lanes per slice) compiled into the
+// library, S = E * L.  The default shape for each S is chosen by
+// default_shape(); the others are alternatives for the autotuner.
+#define TPRH_SLICE_SHAPES(X) \
+    X(1, 2)                  \
+    X(1, 4)                  \
+    X(1, 8)                  \
+    X(1, 16)                 \
+    X(2, 16)                 \
+    X(4, 16)                 \
+    X(8, 16)                 \
+    X(16, 16)                \
+    X(4, 64)                 \
+    X(8, 32)                 \
+    X(8, 64)                 \
+    X(16, 32)                \
+    X(16, 64)                \
+    X(32, 64)
+
+inline bool shape_compiled(int E, int L) {
+#define TPRH_Q(EE, LL) \
+    if (E == EE && L == LL) return true;
+    TPRH_SLICE_SHAPES(TPRH_Q)
+#undef TPRH_Q
+    return false;
+}
+
+// Default (E, L) for a slice size s; returns false if s is unsupported.
+inline bool default_shape(int s, int* E, int* L) {
+    switch (s) {
+        case 2: *E = 1; *L = 2; return true;
+        case 4: *E = 1; *L = 4; return true;
+        case 8: *E = 1; *L = 8; return true;
+        case 16: *E = 1; *L = 16; return true;
+        case 32: *E = 2; *L = 16; return true;
+        case 64: *E = 4; *L = 16; return true;
+        case 128: *E = 8; *L = 16; return true;
+        case 256: *E = 16; *L = 16; return true;
+        case 512: *E = 16; *L = 32; return true;
+        case 1024: *E = 16; *L = 64; return true;
+        case 2048: *E = 32; *L = 64; return true;
+        default: return false;
+    }
+}
+
+struct BatchArgs {
+    const float* a;
+    const float* b;  // may be null: unit diagonal
+    const float* c;
+    const float* d;
+    float* x;
+    int n;     // equations per system
+    int m;     // slices per system (n / s)
+    int nsys;  // systems
+};
+
+// method 0 = PTPR, 1 = TPR; mode 0 = fast, 1 = exact.
+hipError_t launch_batched(int method, int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
+
+}  // namespace tprh

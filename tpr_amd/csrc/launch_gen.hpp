@@ -1,0 +1,22 @@
+// launch_gen.hpp -- host-side launchers of the generator and the single-system
+// PCR / CR kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tprh {
+
+hipError_t launch_generate(int kind, uint64_t seed, uint64_t sys0, int nsys, int n, float* a, float* b, float* c,
+                           float* d, float* xt, hipStream_t stream);
+hipError_t launch_generate_rhs(uint64_t seed, uint64_t sys, int j0, int nrhs, int n, float* d, hipStream_t stream);
+
+// Largest n the single-workgroup PCR / CR kernels accept.
+constexpr int kSingleMaxN = 4096;
+
+// Batched whole-system PCR (method 2) or CR (method 3): one workgroup per
+// system, nsys systems of n equations back to back; b == nullptr means a unit
+// diagonal.
+hipError_t launch_single(int method, const float* a, const float* b, const float* c, const float* r, float* x, int n,
+                         int nsys, hipStream_t stream);
+
+}  // namespace tprh

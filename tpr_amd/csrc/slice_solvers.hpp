@@ -1,0 +1,447 @@
+// slice_solvers.hpp -- the PTPR and TPR solve of one system by a group of
+// lanes, as device functions shared by the batched, uneven-batch and
+// multi-RHS kernels.
+//
+// Data flow of one solve (all in one kernel, no grid-wide synchronisation):
+//   1. each lane loads E consecutive equations of a/b/c/d (vector loads) and
+//      divides by the diagonal in registers (the 4-argument setter,
+//      src/ptpr.hpp:80-90);
+//   2. stage 1 in registers: PTPR runs the in-slice PCR of src/ptpr.cpp:130-190,
+//      TPR the tree reduction of src/tpr.cpp:123-230; partners in other lanes
+//      arrive by DPP row shifts (L == 16) or ds_bpermute (L > 16);
+//   3. the E_st += s1 * E_ed update (src/ptpr.cpp:193-211) and the reduced
+//      system (src/ptpr.cpp:218-241) go through LDS, one row per slice;
+//   4. stage 2 solves the m-row reduced system in LDS: PCR for PTPR
+//      (src/pcr.cpp:15-89), CR for TPR (src/cr.cpp:23-116);
+//   5. stage 3 in registers: PTPR's one-step substitution (src/ptpr.cpp:263-287)
+//      or TPR's tree substitution (src/tpr.cpp:277-308);
+//   6. each lane stores its E answers.
+// HBM traffic is the algorithmic minimum: 16 B/eq read (12 with a unit
+// diagonal) and 4 B/eq written.
+#pragma once
+#include <float.h>
+
+#include "tpr_device.hpp"
+
+namespace tprh {
+
+// LDS words per system for the slice-boundary exchange and the reduced solve.
+__host__ __device__ constexpr int lds_words_per_system(int m) { return 13 * m; }
+
+// ---------------------------------------------------------------------------
+// Loads and stores of E consecutive floats.
+template <int E>
+__device__ __forceinline__ void load_e(const float* __restrict__ p, float (&v)[E]) {
+    if constexpr (E % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < E / 4; i++) {
+            float4 t = reinterpret_cast<const float4*>(p)[i];
+            v[4 * i + 0] = t.x;
+            v[4 * i + 1] = t.y;
+            v[4 * i + 2] = t.z;
+            v[4 * i + 3] = t.w;
+        }
+    } else if constexpr (E == 2) {
+        float2 t = reinterpret_cast<const float2*>(p)[0];
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = p[i];
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void store_e(float* __restrict__ p, const float (&v)[E]) {
+    if constexpr (E % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < E / 4; i++) {
+            reinterpret_cast<float4*>(p)[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+        }
+    } else if constexpr (E == 2) {
+        reinterpret_cast<float2*>(p)[0] = make_float2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; i++) p[i] = v[i];
+    }
+}
+
+// Loads one lane's E equations and normalises them by the diagonal.
+// b == nullptr means a unit diagonal (the reference's GPU entry points).
+template <int E, int MODE>
+__device__ __forceinline__ void load_system_part(const float* __restrict__ a, const float* __restrict__ b,
+                                                 const float* __restrict__ c, const float* __restrict__ d,
+                                                 size_t off, bool valid, float (&A)[E], float (&C)[E],
+                                                 float (&R)[E]) {
+    if (valid) {
+        load_e<E>(a + off, A);
+        load_e<E>(c + off, C);
+        load_e<E>(d + off, R);
+        if (b != nullptr) {
+            float B[E];
+            load_e<E>(b + off, B);
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                if constexpr (MODE == kExact) {
+                    A[i] = A[i] / B[i];
+                    C[i] = C[i] / B[i];
+                    R[i] = R[i] / B[i];
+                } else {
+                    float rb = fast_rcp(B[i]);
+                    A[i] *= rb;
+                    C[i] *= rb;
+                    R[i] *= rb;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            A[i] = 0.0f;
+            C[i] = 0.0f;
+            R[i] = 0.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PTPR stage 1: in-slice PCR, levels u = U, 2U, ... < S (src/ptpr.cpp:134-190).
+template <int E, int L, int MODE, int U>
+__device__ __forceinline__ void ptpr_stage1_level(float (&A)[E], float (&C)[E], float (&R)[E], int l) {
+    constexpr int S = E * L;
+    if constexpr (U < E) {
+        // Partners inside the lane, except the first / last U equations whose
+        // partners sit in the previous / next lane of the slice.
+        Eq left[U], right[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            left[j].a = shift_up<L, 1>(A[E - U + j], -1.0f, l);
+            left[j].c = shift_up<L, 1>(C[E - U + j], 0.0f, l);
+            left[j].r = shift_up<L, 1>(R[E - U + j], 0.0f, l);
+            right[j].a = shift_down<L, 1>(A[j], 0.0f, l);
+            right[j].c = shift_down<L, 1>(C[j], -1.0f, l);
+            right[j].r = shift_down<L, 1>(R[j], 0.0f, l);
+        }
+        float nA[E], nC[E], nR[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            Eq k{A[j], C[j], R[j]};
+            Eq kl = (j >= U) ? Eq{A[j - U], C[j - U], R[j - U]} : left[j];
+            Eq kr = (j + U < E) ? Eq{A[j + U], C[j + U], R[j + U]} : right[j + U - E];
+            bool hasL = (j >= U) || (l > 0);
+            bool hasR = (j + U < E) || (l < L - 1);
+            Eq o = pcr_update<MODE>(k, kl, kr, hasL, hasR);
+            nA[j] = o.a;
+            nC[j] = o.c;
+            nR[j] = o.r;
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            A[j] = nA[j];
+            C[j] = nC[j];
+            R[j] = nR[j];
+        }
+    } else {
+        // Partner of register j is register j of the lane D lanes away.
+        constexpr int D = U / E;
+        bool hasL = l >= D;
+        bool hasR = l < L - D;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            Eq kl{shift_up<L, D>(A[j], -1.0f, l), shift_up<L, D>(C[j], 0.0f, l), shift_up<L, D>(R[j], 0.0f, l)};
+            Eq kr{shift_down<L, D>(A[j], 0.0f, l), shift_down<L, D>(C[j], -1.0f, l),
+                  shift_down<L, D>(R[j], 0.0f, l)};
+            Eq o = pcr_update<MODE>(Eq{A[j], C[j], R[j]}, kl, kr, hasL, hasR);
+            A[j] = o.a;
+            C[j] = o.c;
+            R[j] = o.r;
+        }
+    }
+    if constexpr (2 * U < S) ptpr_stage1_level<E, L, MODE, 2 * U>(A, C, R, l);
+}
+
+// ---------------------------------------------------------------------------
+// TPR stage 1: the tree reduction (src/tpr.cpp:142-216).  At level p (block
+// size P = 2^p, partner distance U = P/2) only block starts (offset % P == 0)
+// and block ends (offset % P == P-1) change.  Even-offset equations keep their
+// original values in Ao/Co/Ro for stage 3 (the reference's bkup of
+// src/tpr.cpp:128-132); odd offsets end with their stage-1 values
+// (src/tpr.cpp:209-213).
+template <int E, int L, int MODE, int U>
+__device__ __forceinline__ void tpr_stage1_level(float (&A)[E], float (&C)[E], float (&R)[E], int l) {
+    constexpr int S = E * L;
+    constexpr int P = 2 * U;
+    if constexpr (P <= E) {
+        // Blocks lie inside one lane.  The first block start needs register
+        // E-U of the previous lane; the last block end needs register U-1 of
+        // the next lane.
+        Eq left{shift_up<L, 1>(A[E - U], -1.0f, l), shift_up<L, 1>(C[E - U], 0.0f, l),
+                shift_up<L, 1>(R[E - U], 0.0f, l)};
+        Eq right{shift_down<L, 1>(A[U - 1], 0.0f, l), shift_down<L, 1>(C[U - 1], -1.0f, l),
+                 shift_down<L, 1>(R[U - 1], 0.0f, l)};
+        float nA[E], nC[E], nR[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            nA[j] = A[j];
+            nC[j] = C[j];
+            nR[j] = R[j];
+        }
+#pragma unroll
+        for (int j = 0; j < E; j += P) {
+            // block start j
+            {
+                Eq kl = (j >= U) ? Eq{A[j - U], C[j - U], R[j - U]} : left;
+                Eq kr{A[j + U], C[j + U], R[j + U]};
+                bool hasL = (j >= U) || (l > 0);
+                Eq o = pcr_update<MODE>(Eq{A[j], C[j], R[j]}, kl, kr, hasL, true);
+                nA[j] = o.a;
+                nC[j] = o.c;
+                nR[j] = o.r;
+            }
+            // block end j + P - 1
+            {
+                const int k = j + P - 1;
+                Eq kl{A[k - U], C[k - U], R[k - U]};
+                Eq kr = (k + U < E) ? Eq{A[k + U], C[k + U], R[k + U]} : right;
+                bool hasR = (k + U < E) || (l < L - 1);
+                Eq o = pcr_update<MODE>(Eq{A[k], C[k], R[k]}, kl, kr, true, hasR);
+                nA[k] = o.a;
+                nC[k] = o.c;
+                nR[k] = o.r;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            A[j] = nA[j];
+            C[j] = nC[j];
+            R[j] = nR[j];
+        }
+    } else {
+        // Blocks span Q = P/E lanes.  Starts are register 0 of lanes with
+        // l % Q == 0, ends register E-1 of lanes with l % Q == Q-1; partners
+        // sit D = U/E lanes away in the same register.
+        constexpr int D = U / E;
+        constexpr int Q = P / E;
+        const int lq = l & (Q - 1);
+        bool hasL = l >= D;
+        bool hasR = l < L - D;
+        Eq s0{A[0], C[0], R[0]};
+        Eq sl{shift_up<L, D>(s0.a, -1.0f, l), shift_up<L, D>(s0.c, 0.0f, l), shift_up<L, D>(s0.r, 0.0f, l)};
+        Eq sr{shift_down<L, D>(s0.a, 0.0f, l), shift_down<L, D>(s0.c, -1.0f, l),
+              shift_down<L, D>(s0.r, 0.0f, l)};
+        Eq e0{A[E - 1], C[E - 1], R[E - 1]};
+        Eq el{shift_up<L, D>(e0.a, -1.0f, l), shift_up<L, D>(e0.c, 0.0f, l), shift_up<L, D>(e0.r, 0.0f, l)};
+        Eq er{shift_down<L, D>(e0.a, 0.0f, l), shift_down<L, D>(e0.c, -1.0f, l),
+              shift_down<L, D>(e0.r, 0.0f, l)};
+        Eq so = pcr_update<MODE>(s0, sl, sr, hasL, hasR);
+        Eq eo = pcr_update<MODE>(e0, el, er, hasL, hasR);
+        if (lq == 0) {
+            A[0] = so.a;
+            C[0] = so.c;
+            R[0] = so.r;
+        }
+        if (lq == Q - 1) {
+            A[E - 1] = eo.a;
+            C[E - 1] = eo.c;
+            R[E - 1] = eo.r;
+        }
+    }
+    if constexpr (P < S) tpr_stage1_level<E, L, MODE, P>(A, C, R, l);
+}
+
+// TPR stage 3 (src/tpr.cpp:277-308): tree back-substitution.  X[E-1] of the
+// slice's last lane holds x_ed on entry; xprev is x[st-1] (0 for slice 0).
+// Coefficients: even offsets use the originals (Ao...), odd offsets the
+// stage-1 values (A...).
+template <int E, int L, int MODE, int U>
+__device__ __forceinline__ void tpr_stage3_level(const float (&A)[E], const float (&C)[E], const float (&R)[E],
+                                                 const float (&Ao)[E], const float (&Co)[E],
+                                                 const float (&Ro)[E], float (&X)[E], float xprev, int l) {
+    auto coef = [&](int j, float& ca, float& cc, float& cr) {
+        // offset parity: E even -> parity of j; E == 1 -> parity of l
+        bool even = (E % 2 == 0) ? ((j & 1) == 0) : ((l & 1) == 0);
+        ca = even ? Ao[j] : A[j];
+        cc = even ? Co[j] : C[j];
+        cr = even ? Ro[j] : R[j];
+    };
+    auto upd = [&](float ca, float cc, float cr, float xl, float xr) -> float {
+        if constexpr (MODE == kExact) {
+            return cr - ca * xl - cc * xr;
+        } else {
+            return __builtin_fmaf(-cc, xr, __builtin_fmaf(-ca, xl, cr));
+        }
+    };
+    if constexpr (U >= E) {
+        constexpr int D = U / E;
+        float xl = shift_up<L, D>(X[E - 1], xprev, l);
+        float xr = shift_down<L, D>(X[E - 1], 0.0f, l);
+        if (((l + 1) & (2 * D - 1)) == D) {
+            float ca, cc, cr;
+            coef(E - 1, ca, cc, cr);
+            X[E - 1] = upd(ca, cc, cr, xl, xr);
+        }
+        if constexpr (U > 1) tpr_stage3_level<E, L, MODE, U / 2>(A, C, R, Ao, Co, Ro, X, xprev, l);
+    } else {
+        // In-lane levels U = E/2 .. 1; x of the previous lane's last equation
+        // is final by now.
+        float xleft = shift_up<L, 1>(X[E - 1], xprev, l);
+#pragma unroll
+        for (int u = E / 2; u >= 1; u /= 2) {
+#pragma unroll
+            for (int j = u - 1; j < E; j += 2 * u) {
+                float xl = (j >= u) ? X[j - u] : xleft;
+                float ca, cc, cr;
+                coef(j, ca, cc, cr);
+                X[j] = upd(ca, cc, cr, xl, X[j + u]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The middle of the solve, shared by PTPR and TPR: E_st update, reduced
+// system, stage 2.  On return xr[j] holds x at the end of slice j.
+// lds: lds_words_per_system(m) floats for this system.
+// (eA, eC, eR) is E_ed in slice lane L-1, (sA, sC, sR) is E_st in slice lane 0.
+template <int METHOD>  // 0 = PTPR (PCR stage 2), 1 = TPR (CR stage 2)
+__device__ __forceinline__ const float* reduced_solve(float* lds, int m, int j, int l, int L, int t,
+                                                      float eA, float eC, float eR, float sA, float sC,
+                                                      float sR) {
+    float* ed_a = lds;
+    float* ed_c = ed_a + m;
+    float* ed_r = ed_c + m;
+    float* st_a = ed_r + m;
+    float* st_c = st_a + m;
+    float* st_r = st_c + m;
+    float* ra0 = st_r + m;
+    float* rc0 = ra0 + m;
+    float* rr0 = rc0 + m;
+    float* ra1 = rr0 + m;
+    float* rc1 = ra1 + m;
+    float* rr1 = rc1 + m;
+    float* xo = rr1 + m;
+
+    if (l == L - 1) {
+        ed_a[j] = eA;
+        ed_c[j] = eC;
+        ed_r[j] = eR;
+    }
+    __syncthreads();
+    if (l == 0) {
+        // E_st += s1 * E_ed, src/ptpr.cpp:193-211 (src/tpr.cpp:327-340)
+        float c_ed = ed_c[j];
+        float s1 = (fabsf(c_ed) < FLT_EPSILON) ? -1.0f : -sC / c_ed;
+        st_a[j] = sA + s1 * ed_a[j];
+        st_c[j] = s1;
+        st_r[j] = sR + s1 * ed_r[j];
+    }
+    __syncthreads();
+    if (t < m) {
+        // reduced row t from (E_ed(t), E_st(t+1)), src/ptpr.cpp:218-241
+        if (t < m - 1) {
+            float ak = ed_a[t], ck = ed_c[t], rk = ed_r[t];
+            float akr = st_a[t + 1], ckr = st_c[t + 1], rkr = st_r[t + 1];
+            float inv = 1.0f / (1.0f - akr * ck);
+            ra0[t] = inv * ak;
+            rc0[t] = -inv * ckr * ck;
+            rr0[t] = inv * (rk - rkr * ck);
+        } else {
+            ra0[t] = ed_a[t];
+            rc0[t] = ed_c[t];
+            rr0[t] = ed_r[t];
+        }
+    }
+    __syncthreads();
+    if constexpr (METHOD == 0) {
+        return lds_pcr(ra0, rc0, rr0, ra1, rc1, rr1, m, t, m);
+    } else {
+        lds_cr(ra0, rc0, rr0, xo, m, t, m);
+        return xo;
+    }
+}
+
+// Scratch of one system's middle step, read back by stage 3.
+struct SliceEnd {
+    float a, c, r;  // E_ed after stage 1
+};
+
+__device__ __forceinline__ SliceEnd slice_end(const float* lds, int m, int j) {
+    return SliceEnd{lds[j], lds[m + j], lds[2 * m + j]};
+}
+
+// ---------------------------------------------------------------------------
+// Whole PTPR solve of one system by its lanes.  sys_valid == false lanes run
+// the same instruction stream on zeros (every lane must take part in the
+// cross-lane exchanges and the barriers) and store nothing.
+template <int E, int L, int MODE>
+__device__ __forceinline__ void ptpr_solve_lanes(const float* __restrict__ a, const float* __restrict__ b,
+                                                 const float* __restrict__ c, const float* __restrict__ d,
+                                                 float* __restrict__ x, size_t base, int m, int t,
+                                                 bool sys_valid, float* lds) {
+    const int l = t % L;
+    const int j = t / L;
+    const size_t off = base + (size_t)t * E;
+    float A[E], C[E], R[E];
+    load_system_part<E, MODE>(a, b, c, d, off, sys_valid, A, C, R);
+
+    ptpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+
+    const float* xr = reduced_solve<0>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+
+    // stage 3, src/ptpr.cpp:263-287 (E_st keeps its pre-update values in A/C/R)
+    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+    const float xe = xr[j];
+    SliceEnd se = slice_end(lds, m, j);
+    float key;
+    if constexpr (MODE == kExact) {
+        key = 1.0f / se.c * (se.r - se.a * xprev - xe);
+    } else {
+        key = fast_rcp(se.c) * (__builtin_fmaf(-se.a, xprev, se.r) - xe);
+    }
+    if (se.c == 0.0f) key = 0.0f;
+    float X[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        if constexpr (MODE == kExact) {
+            X[i] = R[i] - A[i] * xprev - C[i] * key;
+        } else {
+            X[i] = __builtin_fmaf(-C[i], key, __builtin_fmaf(-A[i], xprev, R[i]));
+        }
+    }
+    if (l == L - 1) X[E - 1] = xe;
+    if (sys_valid) store_e<E>(x + off, X);
+}
+
+// Whole TPR solve of one system by its lanes.
+template <int E, int L, int MODE>
+__device__ __forceinline__ void tpr_solve_lanes(const float* __restrict__ a, const float* __restrict__ b,
+                                                const float* __restrict__ c, const float* __restrict__ d,
+                                                float* __restrict__ x, size_t base, int m, int t,
+                                                bool sys_valid, float* lds) {
+    const int l = t % L;
+    const int j = t / L;
+    const size_t off = base + (size_t)t * E;
+    float A[E], C[E], R[E];
+    load_system_part<E, MODE>(a, b, c, d, off, sys_valid, A, C, R);
+    float Ao[E], Co[E], Ro[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        Ao[i] = A[i];
+        Co[i] = C[i];
+        Ro[i] = R[i];
+    }
+
+    tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+
+    const float* xr = reduced_solve<1>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+
+    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+    float X[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) X[i] = 0.0f;
+    X[E - 1] = (l == L - 1) ? xr[j] : 0.0f;
+    tpr_stage3_level<E, L, MODE, (E * L) / 2>(A, C, R, Ao, Co, Ro, X, xprev, l);
+    if (sys_valid) store_e<E>(x + off, X);
+}
+
+}  // namespace tprh
