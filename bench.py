@@ -1,0 +1,202 @@
+"""Benchmark: batched PTPR on one MI355X.
+
+Workload (one step): one batched PTPR solve of 4096 independent fp32 systems,
+N = 1024 each, slice size S = 256, diagonally dominant generator, inputs
+resident in HBM.  Throughput is systems/s over the whole timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--method ptpr|tpr]
+                    [--arith fast|exact] [--graph] [--no-cpu]
+
+The batch is kept in ROT buffer sets (each 83.9 MB with x) that the steps
+cycle through, more than the 256 MB Infinity Cache holds, so every step reads
+its inputs from HBM rather than from the cache.
+
+Roofline: algorithmic bytes per launch = 20 B per equation (read a, b, c, d;
+write x) x 1024 x 4096 = 83,886,080 B, divided by the kernel's average
+duration from HIP events on the launch stream, against the 8 TB/s HBM peak.
+
+CPU baseline: the reference's own PTPR (oracle/_ref, compiled from its
+sources; the oracle's restatement where that build is absent), one solver
+per thread over the systems, on this host's cores, on a bounded sample of the
+same batch; the GPU answers of that sample are compared with it (rel-L2).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "systems/s & % HBM BW at rel L2≤1e-5 vs plain TPR; p50/p99 serve latency"
+HBM_PEAK_GBS = 8000.0
+NSYS, N, S = 4096, 1024, 256
+ROT = 4  # buffer sets cycled through (4 x 83.9 MB > 256 MB cache)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--method", default="ptpr", choices=["ptpr", "tpr"])
+    ap.add_argument("--arith", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--graph", action="store_true", help="replay the timed steps from a captured HIP graph")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+def gpu_leg(args):
+    import torch
+
+    import tpr_amd
+    from tpr_amd import batched as B
+
+    torch.cuda.set_device(0)
+    sets = []
+    for r in range(ROT):
+        t = B.alloc_batch(NSYS, N, with_xt=(r == 0))
+        B.generate("dominant", 20240 + r, 0, NSYS, N, t["a"], t["b"], t["c"], t["d"], t.get("xt"))
+        sets.append(t)
+    torch.cuda.synchronize()
+    def step(i):
+        # launched on the current stream (the capture stream while a graph is captured)
+        t = sets[i % ROT]
+        tpr_amd.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+
+    graph = None
+    if args.graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for i in range(args.steps):
+                step(i)
+        graph.replay()
+        torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    # timed region: synchronize, K steps, synchronize
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    if graph is not None:
+        graph.replay()
+    else:
+        for i in range(args.steps):
+            step(i)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1)
+
+    # per-launch kernel duration: events around each launch, same stream
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(20)]
+    for i, (e0, e1) in enumerate(evs):
+        e0.record(stream)
+        step(i)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    per_launch = sorted(e0.elapsed_time(e1) for e0, e1 in evs)
+    kern_ms = per_launch[len(per_launch) // 2]
+
+    # answers of set 0 for the accuracy check (solved last above for i % ROT == 0)
+    step(0)
+    torch.cuda.synchronize()
+    x0 = sets[0]["x"].cpu().numpy()
+    host = {k: sets[0][k].cpu().numpy() for k in ("a", "b", "c", "d", "xt")}
+    return {"wall_s": wall, "event_ms": ev_ms, "kernel_ms_median": kern_ms, "kernel_ms_min": per_launch[0],
+            "x0": x0, "host": host}
+
+
+def cpu_leg(args, host, x_gpu):
+    import numpy as np
+
+    import oracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    solve = O.ref_solve_batch if kind == "reference" else O.solve_batch
+    threads = O.max_threads()
+    n_sample = 512  # systems per timed call (a sample of the batch)
+    sl = slice(0, n_sample * N)
+    a, b, c, d = (host[k][sl] for k in "abcd")
+    solve(args.method, a, b, c, d, N, S, 1)  # warm
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        x_cpu = solve(args.method, a, b, c, d, N, S, 1)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds:
+            break
+    cpu_sys_s = reps * n_sample / el
+    # parity of the GPU answers on the whole set 0 against the CPU PTPR / TPR
+    x_ref = solve(args.method, host["a"], host["b"], host["c"], host["d"], N, S, 1)
+    xg = x_gpu.reshape(NSYS, N).astype(np.float64)
+    xr = x_ref.reshape(NSYS, N).astype(np.float64)
+    rel = np.linalg.norm(xg - xr, axis=1) / np.linalg.norm(xr, axis=1)
+    bitexact = bool(np.array_equal(x_gpu.view(np.uint32), x_ref.view(np.uint32)))
+    xt = host["xt"].reshape(NSYS, N).astype(np.float64)
+    err_t = np.linalg.norm(xg - xt, axis=1) / np.linalg.norm(xt, axis=1)
+    base = {"value": round(cpu_sys_s, 1), "unit": "systems/s", "cores": threads, "kind": kind,
+            "sample": "%d of the %d systems (N=%d, S=%d), solved %d times in %.1f s, one %s solver per thread "
+                      "(OpenMP over systems)" % (n_sample, NSYS, N, S, reps, el, args.method.upper())}
+    acc = {"rel_l2_vs_cpu_max": float(rel.max()), "rel_l2_vs_cpu_mean": float(rel.mean()),
+           "bit_identical_to_cpu": bitexact, "rel_l2_vs_truth_max": float(err_t.max()), "cpu_check": kind}
+    return base, acc
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (systems are independent and "
+              "would be sharded by count across GPUs); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    g = gpu_leg(args)
+    ms_per_step = g["event_ms"] / args.steps
+    value = NSYS / (ms_per_step / 1e3)
+    alg_bytes = 20 * N * NSYS
+    achieved = alg_bytes / (g["kernel_ms_median"] / 1e3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_config1.json" % args.method)
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "systems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched %s: %d fp32 systems x N=%d, S=%d, diagonally dominant generator, "
+                               "inputs resident in HBM" % (args.method.upper(), NSYS, N, S),
+                   "nsys": NSYS, "n": N, "s": S, "method": args.method, "arith": args.arith,
+                   "graph": bool(args.graph), "buffer_sets": ROT},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "kernel_ms_median": round(g["kernel_ms_median"], 5),
+        "kernel_ms_min": round(g["kernel_ms_min"], 5),
+        "achieved_gbs_whole_job": round(alg_bytes / (ms_per_step / 1e3) / 1e9, 1),
+    }
+    if not args.no_cpu:
+        base, acc = cpu_leg(args, g["host"], g["x0"])
+        out["cpu_baseline"] = base
+        out["accuracy"] = acc
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -74,7 +74,7 @@ def test_generator_bit_identical(kind, n, nsys):
 # batched PTPR / TPR, default slice shapes
 
 CASES = [(16, 2), (16, 4), (64, 8), (256, 16), (512, 32), (1024, 64), (2048, 128), (1024, 256),
-         (2048, 512), (4096, 1024), (8192, 2048), (256, 256), (1024, 4), (16384, 256), (4096, 16)]
+         (2048, 512), (4096, 1024), (8192, 2048), (256, 256), (1024, 4), (16384, 256), (1024, 16)]
 
 
 @pytest.mark.parametrize("n,s", CASES)
@@ -145,7 +145,7 @@ def test_config1_full_size(method):
 # edge cases the reference exercises or documents
 
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
-@pytest.mark.parametrize("n,s", [(64, 8), (1024, 256), (2048, 16)])
+@pytest.mark.parametrize("n,s", [(64, 8), (1024, 256), (512, 16)])
 def test_branch_systems(method, n, s):
     """c = 0 at slice ends: s1 = -1 (src/ptpr.cpp:196-200) and key = 0
     (src/ptpr.cpp:276-279) in every slice."""
