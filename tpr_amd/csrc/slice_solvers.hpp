@@ -161,6 +161,69 @@ __device__ __forceinline__ void ptpr_stage1_level(float (&A)[E], float (&C)[E], 
 }
 
 // ---------------------------------------------------------------------------
+// PTPR stage 1, Fast policy, slices of one DPP row (L == 16).  Ap = -a and
+// Cp = -c (negated off-diagonals); every cross-lane partner is folded into a
+// fused DPP instruction (see dpp_den_rn / dpp_keep_mul_*).  In the negated
+// form the PCR update of src/ptpr.cpp:155-168 reads
+//   den = 1 - C'kl A'k - A'kr C'k,  inv = 1/den,
+//   A'  = A'kl (inv A'k),  C' = C'kr (inv C'k),  r = inv (rk + rkl A'k + rkr C'k),
+// and the upper / lower formulas (:140-152, :171-182) are the same expression
+// with the missing partner's C'kl, rkl (resp. A'kr, rkr) zero and A'kl
+// (resp. C'kr) one.
+template <int E, int U>
+__device__ __forceinline__ void ptpr_stage1_fast16(float (&Ap)[E], float (&Cp)[E], float (&R)[E]) {
+    constexpr int S = E * 16;
+    if constexpr (U < E) {
+        float nA[E], nC[E], nR[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const bool lx = j < U;       // left partner in the previous lane (register E-U+j)
+            const bool rx = j + U >= E;  // right partner in the next lane (register j+U-E)
+            float ak = Ap[j], ck = Cp[j];
+            float den, rn;
+            if (!lx && !rx) {
+                den = __builtin_fmaf(-Ap[j + U], ck, __builtin_fmaf(-Cp[j - U], ak, 1.0f));
+                rn = __builtin_fmaf(R[j + U], ck, __builtin_fmaf(R[j - U], ak, R[j]));
+            } else if (lx) {
+                den = __builtin_fmaf(-Ap[j + U], ck, 1.0f);
+                rn = __builtin_fmaf(R[j + U], ck, R[j]);
+                dpp_den_rn_up<1>(den, rn, Cp[lx ? E - U + j : 0], R[lx ? E - U + j : 0], ak);
+            } else {
+                den = __builtin_fmaf(-Cp[j - U], ak, 1.0f);
+                rn = __builtin_fmaf(R[j - U], ak, R[j]);
+                dpp_den_rn_dn<1>(den, rn, Ap[rx ? j + U - E : 0], R[rx ? j + U - E : 0], ck);
+            }
+            const float inv = fast_rcp(den);
+            float u = inv * ak, v = inv * ck;
+            nA[j] = lx ? dpp_keep_mul_up<1>(Ap[lx ? E - U + j : 0], u) : Ap[lx ? 0 : j - U] * u;
+            nC[j] = rx ? dpp_keep_mul_dn<1>(Cp[rx ? j + U - E : 0], v) : Cp[rx ? 0 : j + U] * v;
+            nR[j] = inv * rn;
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            Ap[j] = nA[j];
+            Cp[j] = nC[j];
+            R[j] = nR[j];
+        }
+    } else {
+        constexpr int D = U / E;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const float ak = Ap[j], ck = Cp[j];
+            float den = 1.0f, rn = R[j];
+            dpp_den_rn<D>(den, rn, ck, ak, R[j], R[j], ak, ck);
+            const float inv = fast_rcp(den);
+            const float nA = dpp_keep_mul_up<D>(ak, inv * ak);
+            const float nC = dpp_keep_mul_dn<D>(ck, inv * ck);
+            R[j] = inv * rn;
+            Ap[j] = nA;
+            Cp[j] = nC;
+        }
+    }
+    if constexpr (2 * U < S) ptpr_stage1_fast16<E, 2 * U>(Ap, Cp, R);
+}
+
+// ---------------------------------------------------------------------------
 // TPR stage 1: the tree reduction (src/tpr.cpp:142-216).  At level p (block
 // size P = 2^p, partner distance U = P/2) only block starts (offset % P == 0)
 // and block ends (offset % P == P-1) change.  Even-offset equations keep their
@@ -384,7 +447,21 @@ __device__ __forceinline__ void ptpr_solve_lanes(const float* __restrict__ a, co
     float A[E], C[E], R[E];
     load_system_part<E, MODE>(a, b, c, d, off, sys_valid, A, C, R);
 
-    ptpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+    if constexpr (MODE == kFast && L == 16) {
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            A[i] = -A[i];
+            C[i] = -C[i];
+        }
+        ptpr_stage1_fast16<E, 1>(A, C, R);
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            A[i] = -A[i];
+            C[i] = -C[i];
+        }
+    } else {
+        ptpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+    }
 
     const float* xr = reduced_solve<0>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
 

@@ -82,6 +82,74 @@ __device__ __forceinline__ float shift_down(float v, float bnd, int l) {
 // Reciprocal used by the Fast policy: v_rcp_f32 (1 ulp).
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// ---------------------------------------------------------------------------
+// Fused DPP forms for slices of L == 16 lanes (one DPP row), Fast policy.
+//
+// The fast PTPR stage 1 keeps the off-diagonals negated (A = -a, C = -c) so
+// that every virtual boundary partner is either a zero (C and r of the left
+// virtual row, A and r of the right one: DPP zero-fill, bound_ctrl) or a
+// multiplicative identity (A = 1 of the left row, C = 1 of the right row: the
+// DPP lane is disabled and the destination keeps its value).  Each exchange
+// then folds into the ALU instruction that consumes it.  The compiler's DPP
+// combiner does not form these (it leaves an FMA or an identity multiply
+// unfused), so they are written out here.  "s_nop 1" gives the two wait
+// states GFX9 requires between a VALU write of a VGPR and a DPP read of it.
+// The accumulators of the multi-instruction blocks are early-clobber: they are
+// written before the last input is read, so they must not share a register
+// with any input (rn starts equal to r, which is also an input).
+
+// den -= Cs(lane-D) * ak;  den -= As(lane+D) * ck;
+// rn  += Rs(lane-D) * ak;  rn  += Qs(lane+D) * ck      (zero past the row ends)
+template <int D>
+__device__ __forceinline__ void dpp_den_rn(float& den, float& rn, float Cs, float As, float Rs, float Qs, float ak,
+                                           float ck) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %[den], -%[cs], %[ak] row_shr:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %[den], -%[as], %[ck] row_shl:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %[rn], %[rs], %[ak] row_shr:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %[rn], %[qs], %[ck] row_shl:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : [den] "+&v"(den), [rn] "+&v"(rn)
+        : [cs] "v"(Cs), [as] "v"(As), [rs] "v"(Rs), [qs] "v"(Qs), [ak] "v"(ak), [ck] "v"(ck), [d] "i"(D));
+}
+
+// Left partner only: den -= Cs(lane-D) * ak;  rn += Rs(lane-D) * ak.
+template <int D>
+__device__ __forceinline__ void dpp_den_rn_up(float& den, float& rn, float Cs, float Rs, float ak) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %[den], -%[cs], %[ak] row_shr:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %[rn], %[rs], %[ak] row_shr:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : [den] "+&v"(den), [rn] "+&v"(rn)
+        : [cs] "v"(Cs), [rs] "v"(Rs), [ak] "v"(ak), [d] "i"(D));
+}
+
+// Right partner only: den -= As(lane+D) * ck;  rn += Rs(lane+D) * ck.
+template <int D>
+__device__ __forceinline__ void dpp_den_rn_dn(float& den, float& rn, float As, float Rs, float ck) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %[den], -%[as], %[ck] row_shl:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %[rn], %[rs], %[ck] row_shl:%[d] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : [den] "+&v"(den), [rn] "+&v"(rn)
+        : [as] "v"(As), [rs] "v"(Rs), [ck] "v"(ck), [d] "i"(D));
+}
+
+// u = As(lane-D) * u, or u unchanged in the first D lanes of the row.
+template <int D>
+__device__ __forceinline__ float dpp_keep_mul_up(float As, float u) {
+    asm("s_nop 1\n\tv_mul_f32_dpp %[u], %[as], %[u] row_shr:%[d] row_mask:0xf bank_mask:0xf"
+        : [u] "+v"(u)
+        : [as] "v"(As), [d] "i"(D));
+    return u;
+}
+
+// v = Cs(lane+D) * v, or v unchanged in the last D lanes of the row.
+template <int D>
+__device__ __forceinline__ float dpp_keep_mul_dn(float Cs, float v) {
+    asm("s_nop 1\n\tv_mul_f32_dpp %[v], %[cs], %[v] row_shl:%[d] row_mask:0xf bank_mask:0xf"
+        : [v] "+v"(v)
+        : [cs] "v"(Cs), [d] "i"(D));
+    return v;
+}
+
 // One equation of a tridiagonal system with unit diagonal:
 //   a * x[k - u] + x[k] + c * x[k + u] = r.
 struct Eq {
