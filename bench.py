@@ -5,15 +5,19 @@ N = 1024 each, slice size S = 256, diagonally dominant generator, inputs
 resident in HBM.  Throughput is systems/s over the whole timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--method ptpr|tpr]
-                    [--arith fast|exact] [--graph] [--no-cpu]
+                    [--arith fast|exact] [--eager] [--no-cpu]
 
 The batch is kept in ROT buffer sets (each 83.9 MB with x) that the steps
 cycle through, more than the 256 MB Infinity Cache holds, so every step reads
 its inputs from HBM rather than from the cache.
 
+The K timed steps are captured once into a HIP graph and replayed, so the
+GPU runs them back to back (--eager launches them one by one from Python).
+
 Roofline: algorithmic bytes per launch = 20 B per equation (read a, b, c, d;
 write x) x 1024 x 4096 = 83,886,080 B, divided by the kernel's average
-duration from HIP events on the launch stream, against the 8 TB/s HBM peak.
+duration (HIP events on the launch stream around the K launches, / K),
+against the 8 TB/s HBM peak.
 
 CPU baseline: the reference's own PTPR (oracle/_ref, compiled from its
 sources; the oracle's restatement where that build is absent), one solver
@@ -42,9 +46,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--method", default="ptpr", choices=["ptpr", "tpr"])
     ap.add_argument("--arith", default="fast", choices=["fast", "exact"])
-    ap.add_argument("--graph", action="store_true", help="replay the timed steps from a captured HIP graph")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the timed steps one by one from Python instead of replaying a captured HIP graph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     return ap.parse_args()
 
 
@@ -55,23 +61,28 @@ def gpu_leg(args):
     from tpr_amd import batched as B
 
     torch.cuda.set_device(0)
+    shape = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
     sets = []
     for r in range(ROT):
         t = B.alloc_batch(NSYS, N, with_xt=(r == 0))
         B.generate("dominant", 20240 + r, 0, NSYS, N, t["a"], t["b"], t["c"], t["d"], t.get("xt"))
         sets.append(t)
     torch.cuda.synchronize()
+
     def step(i):
         # launched on the current stream (the capture stream while a graph is captured)
         t = sets[i % ROT]
-        tpr_amd.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith)
+        tpr_amd.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith,
+                              shape=shape)
 
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
 
     graph = None
-    if args.graph:
+    if not args.eager:
+        # the K timed steps are one captured HIP graph: the GPU runs the
+        # launches back to back without host launch latency between them
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             for i in range(args.steps):
@@ -96,23 +107,12 @@ def gpu_leg(args):
     wall = time.perf_counter() - t0
     ev_ms = ev0.elapsed_time(ev1)
 
-    # per-launch kernel duration: events around each launch, same stream
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(20)]
-    for i, (e0, e1) in enumerate(evs):
-        e0.record(stream)
-        step(i)
-        e1.record(stream)
-    torch.cuda.synchronize()
-    per_launch = sorted(e0.elapsed_time(e1) for e0, e1 in evs)
-    kern_ms = per_launch[len(per_launch) // 2]
-
-    # answers of set 0 for the accuracy check (solved last above for i % ROT == 0)
+    # answers of set 0 for the accuracy check
     step(0)
     torch.cuda.synchronize()
     x0 = sets[0]["x"].cpu().numpy()
     host = {k: sets[0][k].cpu().numpy() for k in ("a", "b", "c", "d", "xt")}
-    return {"wall_s": wall, "event_ms": ev_ms, "kernel_ms_median": kern_ms, "kernel_ms_min": per_launch[0],
-            "x0": x0, "host": host}
+    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host}
 
 
 def cpu_leg(args, host, x_gpu):
@@ -162,7 +162,10 @@ def main():
     ms_per_step = g["event_ms"] / args.steps
     value = NSYS / (ms_per_step / 1e3)
     alg_bytes = 20 * N * NSYS
-    achieved = alg_bytes / (g["kernel_ms_median"] / 1e3) / 1e9
+    # every step is one launch of the solver kernel, so the kernel's average
+    # duration over the timed region is the event time divided by K
+    kernel_ms = ms_per_step
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_%s_config1.json" % args.method)
     if os.path.exists(tf):
@@ -184,12 +187,11 @@ def main():
         "config": {"workload": "batched %s: %d fp32 systems x N=%d, S=%d, diagonally dominant generator, "
                                "inputs resident in HBM" % (args.method.upper(), NSYS, N, S),
                    "nsys": NSYS, "n": N, "s": S, "method": args.method, "arith": args.arith,
-                   "graph": bool(args.graph), "buffer_sets": ROT},
+                   "graph": not args.eager, "buffer_sets": ROT, "shape": args.shape},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "kernel_ms_median": round(g["kernel_ms_median"], 5),
-        "kernel_ms_min": round(g["kernel_ms_min"], 5),
-        "achieved_gbs_whole_job": round(alg_bytes / (ms_per_step / 1e3) / 1e9, 1),
+        "kernel_ms_avg": round(kernel_ms, 5),
+        "wall_s": round(g["wall_s"], 4),
     }
     if not args.no_cpu:
         base, acc = cpu_leg(args, g["host"], g["x0"])

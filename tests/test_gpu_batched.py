@@ -280,3 +280,48 @@ def test_nstab_matrices(golden, k):
         for method in ("ptpr", "tpr"):
             x = gpu_solve(a, b, c, d, 512, s, method, "exact")
             assert bits_equal(x, golden["ref"]["eq%d_s%d_%s" % (k, s, method)]), (k, s, method)
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+@pytest.mark.parametrize("n,s,nsys", [(1024, 256, 4097), (256, 16, 1001), (64, 8, 333), (512, 2, 77)])
+def test_ragged_batches(method, arith, n, s, nsys):
+    """Batch sizes that leave the last workgroup partly empty."""
+    say("ragged", method, arith, n, s, nsys)
+    t = B.alloc_batch(nsys, n)
+    B.generate("dominant", 77, 5, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
+    B.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method=method, arith=arith)
+    torch.cuda.synchronize()
+    a, b, c, d = (t[k].cpu().numpy() for k in "abcd")
+    ref = O.solve_batch(method, a, b, c, d, n, s)
+    if arith == "exact":
+        assert bits_equal(t["x"].cpu().numpy(), ref)
+    else:
+        assert per_system_rel(t["x"].cpu().numpy(), ref, n).max() <= FAST_TOL
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_graph_capture_replays_match_eager(method):
+    """The batched solve captured in a HIP graph (as bench.py times it):
+    every replay, with eager work and synchronisations in between, equals
+    the eager result bit for bit."""
+    n, s, nsys = 1024, 256, 512
+    say("graph", method)
+    t = B.alloc_batch(nsys, n)
+    B.generate("dominant", 5, 0, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
+    eager = torch.empty_like(t["x"])
+    B.solve_batched(t["a"], t["b"], t["c"], t["d"], eager, n, s, method=method)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        B.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method=method)
+    ref = eager.cpu().numpy()
+    for rep in range(4):
+        t["x"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert bits_equal(t["x"].cpu().numpy(), ref), rep
+        # eager work between replays
+        B.solve_batched(t["a"], t["b"], t["c"], t["d"], eager, n, s, method=method)
+        torch.cuda.synchronize()

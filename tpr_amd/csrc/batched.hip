@@ -12,47 +12,55 @@
 
 namespace tprh {
 
-template <int METHOD, int E, int L, int MODE, int BLOCK>
+// One workgroup holds sys_per_block systems side by side (or one system of up
+// to BLOCK lanes).  HAS_B: the diagonal is given and divided out in registers
+// (false: unit diagonal, b is not read).
+template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B>
 __global__ __launch_bounds__(BLOCK) void batched_kernel(const float* __restrict__ a, const float* __restrict__ b,
                                                        const float* __restrict__ c, const float* __restrict__ d,
                                                        float* __restrict__ x, int n, int m, int nsys,
-                                                       int sys_per_block) {
+                                                       int lg_lanes) {
     extern __shared__ float lds[];
-    const int T = m * L;  // lanes per system
-    const int sl = threadIdx.x / T;
-    const int t = threadIdx.x - sl * T;
-    const int sys = blockIdx.x * sys_per_block + sl;
-    const bool valid = sys < nsys;
-    const size_t base = valid ? (size_t)sys * (size_t)n : 0;
+    const int sl = threadIdx.x >> lg_lanes;                 // system slot in the workgroup
+    const int t = threadIdx.x & ((1 << lg_lanes) - 1);      // lane inside the system
     float* my = lds + sl * lds_words_per_system(m);
+    const int sys = blockIdx.x * (blockDim.x >> lg_lanes) + sl;
+    const bool valid = sys < nsys;
+    const size_t off = (valid ? (size_t)sys * (size_t)n : 0) + (size_t)t * E;
+    RawPart<E> raw;
+    load_raw<E>(a, HAS_B ? b : nullptr, c, d, off, raw);
     if constexpr (METHOD == 0) {
-        ptpr_solve_lanes<E, L, MODE>(a, b, c, d, x, base, m, t, valid, my);
+        ptpr_solve_lanes<E, L, MODE>(raw, HAS_B, x, off, m, t, valid, my);
     } else {
-        tpr_solve_lanes<E, L, MODE>(a, b, c, d, x, base, m, t, valid, my);
+        tpr_solve_lanes<E, L, MODE>(raw, HAS_B, x, off, m, t, valid, my);
     }
 }
 
 namespace {
 
+template <int METHOD, int E, int L, int MODE, int BLOCK>
+void launch_b(const BatchArgs& p, int grid, int block, size_t shmem, int lg, hipStream_t stream) {
+    if (p.b != nullptr) {
+        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, true>), dim3(grid), dim3(block), shmem, stream,
+                           p.a, p.b, p.c, p.d, p.x, p.n, p.m, p.nsys, lg);
+    } else {
+        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, false>), dim3(grid), dim3(block), shmem, stream,
+                           p.a, p.b, p.c, p.d, p.x, p.n, p.m, p.nsys, lg);
+    }
+}
+
 template <int METHOD, int E, int L, int MODE>
 hipError_t launch_el(const BatchArgs& p, hipStream_t stream) {
-    const int T = p.m * L;
-    int block, spb;
-    if (T <= 64) {
-        block = 64;
-        spb = 64 / T;
-    } else {
-        block = T;
-        spb = 1;
-    }
+    const int T = p.m * L;  // lanes per system, a power of two
+    const int lg = __builtin_ctz((unsigned)T);
+    const int block = T <= 64 ? 64 : T;
+    const int spb = block / T;
     const int grid = (p.nsys + spb - 1) / spb;
     const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(p.m);
     if (block == 64) {
-        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, 64>), dim3(grid), dim3(block), shmem, stream, p.a, p.b,
-                           p.c, p.d, p.x, p.n, p.m, p.nsys, spb);
+        launch_b<METHOD, E, L, MODE, 64>(p, grid, block, shmem, lg, stream);
     } else {
-        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, 1024>), dim3(grid), dim3(block), shmem, stream, p.a,
-                           p.b, p.c, p.d, p.x, p.n, p.m, p.nsys, spb);
+        launch_b<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
     }
     return hipGetLastError();
 }

@@ -66,40 +66,48 @@ __device__ __forceinline__ void store_e(float* __restrict__ p, const float (&v)[
     }
 }
 
-// Loads one lane's E equations and normalises them by the diagonal.
-// b == nullptr means a unit diagonal (the reference's GPU entry points).
+// One lane's E equations as stored in memory (before the division by the
+// diagonal).
+template <int E>
+struct RawPart {
+    float a[E], b[E], c[E], d[E];
+};
+
+// Branch-free: a lane of a system past the end of the batch loads the same
+// lane of system 0 (always present) and never stores, so the loads stay in
+// one basic block and the compiler keeps them where they are issued.
+template <int E>
+__device__ __forceinline__ void load_raw(const float* __restrict__ a, const float* __restrict__ b,
+                                         const float* __restrict__ c, const float* __restrict__ d, size_t off,
+                                         RawPart<E>& p) {
+    load_e<E>(a + off, p.a);
+    load_e<E>(c + off, p.c);
+    load_e<E>(d + off, p.d);
+    if (b != nullptr) load_e<E>(b + off, p.b);
+}
+
+// Divides by the diagonal as the 4-argument setter does (src/ptpr.hpp:80-90);
+// has_b == false means a unit diagonal (the reference's GPU entry points).
 template <int E, int MODE>
-__device__ __forceinline__ void load_system_part(const float* __restrict__ a, const float* __restrict__ b,
-                                                 const float* __restrict__ c, const float* __restrict__ d,
-                                                 size_t off, bool valid, float (&A)[E], float (&C)[E],
-                                                 float (&R)[E]) {
-    if (valid) {
-        load_e<E>(a + off, A);
-        load_e<E>(c + off, C);
-        load_e<E>(d + off, R);
-        if (b != nullptr) {
-            float B[E];
-            load_e<E>(b + off, B);
+__device__ __forceinline__ void normalise(const RawPart<E>& p, bool has_b, float (&A)[E], float (&C)[E],
+                                          float (&R)[E]) {
 #pragma unroll
-            for (int i = 0; i < E; i++) {
-                if constexpr (MODE == kExact) {
-                    A[i] = A[i] / B[i];
-                    C[i] = C[i] / B[i];
-                    R[i] = R[i] / B[i];
-                } else {
-                    float rb = fast_rcp(B[i]);
-                    A[i] *= rb;
-                    C[i] *= rb;
-                    R[i] *= rb;
-                }
+    for (int i = 0; i < E; i++) {
+        if (has_b) {
+            if constexpr (MODE == kExact) {
+                A[i] = p.a[i] / p.b[i];
+                C[i] = p.c[i] / p.b[i];
+                R[i] = p.d[i] / p.b[i];
+            } else {
+                const float rb = fast_rcp(p.b[i]);
+                A[i] = p.a[i] * rb;
+                C[i] = p.c[i] * rb;
+                R[i] = p.d[i] * rb;
             }
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            A[i] = 0.0f;
-            C[i] = 0.0f;
-            R[i] = 0.0f;
+        } else {
+            A[i] = p.a[i];
+            C[i] = p.c[i];
+            R[i] = p.d[i];
         }
     }
 }
@@ -433,19 +441,17 @@ __device__ __forceinline__ SliceEnd slice_end(const float* lds, int m, int j) {
 }
 
 // ---------------------------------------------------------------------------
-// Whole PTPR solve of one system by its lanes.  sys_valid == false lanes run
+// Whole PTPR solve of one system by its lanes, from the raw loads of the
+// lane's E equations at element offset `off`.  sys_valid == false lanes run
 // the same instruction stream on zeros (every lane must take part in the
 // cross-lane exchanges and the barriers) and store nothing.
 template <int E, int L, int MODE>
-__device__ __forceinline__ void ptpr_solve_lanes(const float* __restrict__ a, const float* __restrict__ b,
-                                                 const float* __restrict__ c, const float* __restrict__ d,
-                                                 float* __restrict__ x, size_t base, int m, int t,
-                                                 bool sys_valid, float* lds) {
+__device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
+                                                 size_t off, int m, int t, bool sys_valid, float* lds) {
     const int l = t % L;
     const int j = t / L;
-    const size_t off = base + (size_t)t * E;
     float A[E], C[E], R[E];
-    load_system_part<E, MODE>(a, b, c, d, off, sys_valid, A, C, R);
+    normalise<E, MODE>(raw, has_b, A, C, R);
 
     if constexpr (MODE == kFast && L == 16) {
 #pragma unroll
@@ -491,15 +497,12 @@ __device__ __forceinline__ void ptpr_solve_lanes(const float* __restrict__ a, co
 
 // Whole TPR solve of one system by its lanes.
 template <int E, int L, int MODE>
-__device__ __forceinline__ void tpr_solve_lanes(const float* __restrict__ a, const float* __restrict__ b,
-                                                const float* __restrict__ c, const float* __restrict__ d,
-                                                float* __restrict__ x, size_t base, int m, int t,
-                                                bool sys_valid, float* lds) {
+__device__ __forceinline__ void tpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
+                                                size_t off, int m, int t, bool sys_valid, float* lds) {
     const int l = t % L;
     const int j = t / L;
-    const size_t off = base + (size_t)t * E;
     float A[E], C[E], R[E];
-    load_system_part<E, MODE>(a, b, c, d, off, sys_valid, A, C, R);
+    normalise<E, MODE>(raw, has_b, A, C, R);
     float Ao[E], Co[E], Ro[E];
 #pragma unroll
     for (int i = 0; i < E; i++) {
