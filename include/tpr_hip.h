@@ -85,6 +85,15 @@ tprh_status tprh_tpr_cu(const float *a, const float *c, const float *rhs, float 
 tprh_status tprh_pcr_cu(const float *a, const float *c, const float *rhs, float *x, int n);
 tprh_status tprh_cr_cu(const float *a, const float *c, const float *rhs, float *x, int n);
 
+/* Kernel-only timing of the host-pointer entry points above and of
+ * tprh_solver_solve, as the reference's TPR_PERF build records it with device
+ * events around the launch (src/ptpr.cu:574-587, src/pm.cuh:30-59): when
+ * enabled, each such call records HIP events around its kernel launch and
+ * tprh_last_kernel_ms() returns that duration for the calling thread's last
+ * call (-1 when timing was off). */
+tprh_status tprh_set_kernel_timing(int enable);
+float tprh_last_kernel_ms(void);
+
 /* ---- the solver-class surface --------------------------------------------
  * tprh_solver_create(&h, TPRH_PTPR, n, s)      ~ PTPR t(n, s);
  * tprh_solver_set_system(h, a, NULL, c, rhs)   ~ t.set_tridiagonal_system(a, c, rhs);
@@ -94,7 +103,8 @@ tprh_status tprh_cr_cu(const float *a, const float *c, const float *rhs, float *
  * The solver keeps the host pointers given to set_system (as the reference
  * does) and reads them in solve().  The division by diag happens on the GPU;
  * the caller's arrays are not modified.  methods: TPRH_PTPR, TPRH_TPR (s used)
- * and TPRH_PCR, TPRH_CR (s ignored). */
+ * and TPRH_PCR, TPRH_CR (s ignored).  As in the reference, CR ignores diag
+ * (CR::set_tridiagonal_system, src/cr.cpp:17-21). */
 typedef struct tprh_solver tprh_solver;
 tprh_status tprh_solver_create(tprh_solver **out, tprh_method method, int n, int s);
 tprh_status tprh_solver_set_arith(tprh_solver *h, tprh_arith arith); /* default EXACT */

@@ -1,0 +1,71 @@
+"""The tpr_pm driver (src/pm.cpp:118-268 surface): argument handling on the
+CPU, GPU runs checked against the reference's known answer and the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TPR_PM = os.path.join(ROOT, "tpr_amd", "tpr_pm")
+
+
+def run(args, env=None, timeout=120):
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([TPR_PM] + [str(a) for a in args], capture_output=True, text=True, env=e,
+                          timeout=timeout)
+
+
+def test_usage_error():
+    p = run([])
+    assert p.returncode != 0 and "Usage:" in p.stderr
+
+
+def test_unknown_solver():
+    p = run([1024, 256, 1, "nosuch"])
+    assert p.returncode != 0 and "Solver Not Found" in p.stderr
+
+
+@pytest.mark.parametrize("name", ["ptpr", "TPR", "pcr", "lapack"])
+def test_cpu_solver_names_are_refused(name):
+    p = run([1024, 256, 1, name])
+    assert p.returncode != 0 and "CPU solver" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,s", [("cuptpr", 64), ("CUPTPR", 1024), ("cutpr", 256), ("hipptpr", 128),
+                                    ("hiptpr", 512), ("hippcr", 0), ("hipcr", 0)])
+def test_known_answer_through_the_driver(golden, tmp_path, name, s):
+    """TPR_PM_INPUT=fixed: the system of test/ans1024.txt; x written with %f."""
+    out = tmp_path / "x.txt"
+    p = run([1024, s, 3, name, out], env={"TPR_PM_INPUT": "fixed"})
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.strip().splitlines()
+    assert len(lines) == 3
+    for ln in lines:
+        m = re.fullmatch(r"1024,%d,%s,([0-9.eE+-]+)" % (s, re.escape(name)), ln)
+        assert m, ln
+        assert float(m.group(1)) > 0
+    x = np.array([float(v) for v in out.read_text().strip().split(",")])
+    ans = golden["ans1024"]
+    assert x.size == 1024
+    assert np.all(np.abs(x - ans) <= 1e-2) and np.max((x - ans) ** 2) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_readme_invocation_config0():
+    """README.md:28-30 runs `tpr_pm 2048 512 1000 PTPR`; the GPU solver slot of the same call."""
+    p = run([2048, 512, 1000, "cuptpr"], timeout=300)
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.strip().splitlines()
+    assert len(lines) == 1000
+    ms = [float(ln.split(",")[3]) for ln in lines]
+    assert all(v > 0 for v in ms)
+
+
+@pytest.mark.gpu
+def test_invalid_shape_reports_error():
+    p = run([1000, 256, 1, "cuptpr"])
+    assert p.returncode != 0 and "[Error]" in p.stderr

@@ -33,6 +33,8 @@ SIGNATURES = {
     "tprh_tpr_cu": (_i, [_vp, _vp, _vp, _vp, _i, _i]),
     "tprh_pcr_cu": (_i, [_vp, _vp, _vp, _vp, _i]),
     "tprh_cr_cu": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "tprh_set_kernel_timing": (_i, [_i]),
+    "tprh_last_kernel_ms": (ctypes.c_float, []),
     "tprh_solver_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i]),
     "tprh_solver_set_arith": (_i, [_vp, _i]),
     "tprh_solver_set_system": (_i, [_vp, _vp, _vp, _vp, _vp]),

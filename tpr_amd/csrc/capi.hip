@@ -17,6 +17,8 @@
 namespace {
 
 thread_local std::string g_last_error;
+thread_local bool g_timing = false;
+thread_local float g_last_kernel_ms = -1.0f;
 
 tprh_status fail(tprh_status st, const std::string& msg) {
     g_last_error = msg;
@@ -117,13 +119,30 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     TPRH_HIP(hipMemcpy(dc, c, bytes, hipMemcpyHostToDevice));
     TPRH_HIP(hipMemcpy(dr, rhs, bytes, hipMemcpyHostToDevice));
     if (diag) TPRH_HIP(hipMemcpy(db, diag, bytes, hipMemcpyHostToDevice));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (g_timing) {
+        TPRH_HIP(hipEventCreate(&ev[0]));
+        TPRH_HIP(hipEventCreate(&ev[1]));
+        TPRH_HIP(hipEventRecord(ev[0], nullptr));
+    }
+    hipError_t le;
     if (method == TPRH_PTPR || method == TPRH_TPR) {
         tprh::BatchArgs p{da, diag ? db : nullptr, dc, dr, dx, n, n / s, 1};
-        TPRH_HIP(tprh::launch_batched(method == TPRH_PTPR ? 0 : 1, arith == TPRH_ARITH_EXACT ? 1 : 0, p, E, L,
-                                      nullptr));
+        le = tprh::launch_batched(method == TPRH_PTPR ? 0 : 1, arith == TPRH_ARITH_EXACT ? 1 : 0, p, E, L, nullptr);
     } else {
-        TPRH_HIP(tprh::launch_single(method, da, diag ? db : nullptr, dc, dr, dx, n, 1, nullptr));
+        le = tprh::launch_single(method, da, diag ? db : nullptr, dc, dr, dx, n, 1, nullptr);
     }
+    g_last_kernel_ms = -1.0f;
+    if (g_timing) {
+        hipError_t e1 = hipEventRecord(ev[1], nullptr);
+        if (le == hipSuccess && e1 == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess) {
+            float ms = -1.0f;
+            if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) g_last_kernel_ms = ms;
+        }
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
+    TPRH_HIP(le);
     TPRH_HIP(hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost));
     return TPRH_SUCCESS;
 }
@@ -182,6 +201,13 @@ tprh_status tprh_cr_cu(const float* a, const float* c, const float* rhs, float* 
     return host_path_solve(TPRH_CR, TPRH_ARITH_EXACT, a, nullptr, c, rhs, x, n, 0);
 }
 
+tprh_status tprh_set_kernel_timing(int enable) {
+    g_timing = enable != 0;
+    return TPRH_SUCCESS;
+}
+
+float tprh_last_kernel_ms(void) { return g_last_kernel_ms; }
+
 tprh_status tprh_solver_create(tprh_solver** out, tprh_method method, int n, int s) {
     if (!out) return fail(TPRH_ERROR_INVALID_VALUE, "null handle pointer");
     *out = nullptr;
@@ -220,7 +246,8 @@ tprh_status tprh_solver_set_system(tprh_solver* h, const float* a, const float* 
     if (!h) return fail(TPRH_ERROR_INVALID_VALUE, "null solver");
     if (!a || !c || !rhs) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
     h->a = a;
-    h->diag = diag;
+    // CR::set_tridiagonal_system(a, diag, c, rhs) ignores diag (src/cr.cpp:17-21)
+    h->diag = h->method == TPRH_CR ? nullptr : diag;
     h->c = c;
     h->rhs = rhs;
     h->have_system = true;
