@@ -132,6 +132,40 @@ tprh_status tprh_solve_batched_shape(tprh_method method, tprh_arith arith, const
                                      int E, int L, void *stream);
 int tprh_shape_supported(int E, int L);
 
+/* ---- the general entry point ---------------------------------------------
+ * Everything tprh_solve_batched does, plus:
+ *   - systems larger than one workgroup (n / s * L > 1024 lanes, up to 2048
+ *     slices per system): solved in rounds of `chunk` systems by three
+ *     kernels (stage 1, reduced system, stage 3) through a caller-provided
+ *     workspace of tprh_workspace_bytes(desc) bytes (device memory);
+ *   - coefficients a, b, c stored as fp16 (TPRH_STORE_F16; d and x stay fp32;
+ *     the solve computes in fp32);
+ *   - one iterative-refinement step (refine = 1): x += A^-1 (d - A x) with the
+ *     residual accumulated in FP64 on the GPU; x holds the current solution
+ *     on entry (coefficients must be fp32).
+ * No allocation, no host synchronisation, graph-capturable. */
+typedef enum tprh_storage { TPRH_STORE_F32 = 0, TPRH_STORE_F16 = 1 } tprh_storage;
+
+typedef struct tprh_solve_desc {
+    tprh_method method;     /* TPRH_PTPR or TPRH_TPR */
+    tprh_arith arith;
+    int n, s, nsys;
+    tprh_storage coef;      /* storage of a, b, c */
+    const void *a, *b, *c;  /* b may be NULL: unit diagonal */
+    const float *d;
+    float *x;
+    int refine;             /* 0: x = A^-1 d;  1: x += A^-1 (d - A x) */
+    int E, L;               /* slice shape, 0 and 0 = the default for s */
+    int chunk;              /* systems per round of the multi-workgroup path, 0 = default */
+    void *workspace;        /* device memory, needed when a system exceeds one workgroup */
+    size_t workspace_bytes;
+} tprh_solve_desc;
+
+/* Workspace bytes tprh_solve_ex needs for desc (0 when none); (size_t)-1 if
+ * desc is invalid. */
+size_t tprh_workspace_bytes(const tprh_solve_desc *desc);
+tprh_status tprh_solve_ex(const tprh_solve_desc *desc, void *stream);
+
 /* Synthetic systems with ids sys0 .. sys0+nsys-1 written back to back into
  * device arrays a, b, c, d (and xt unless NULL); bit-identical to the
  * oracle's generator on the host.  Asynchronous on `stream`. */

@@ -97,3 +97,25 @@ def test_python_mirror_validates_arrays():
         t.set_tridiagonal_system(a, a)
     with pytest.raises(tpr_amd.TprError):
         tpr_amd.PTPR(100, 16)
+
+
+def test_workspace_sizing_without_device():
+    import ctypes
+    from tpr_amd import _lib
+    L = _lib.lib()
+    d = _lib.SolveDesc()
+    d.method, d.arith, d.n, d.s, d.nsys = 0, 0, 1024, 256, 10
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 0          # one workgroup per system
+    d.n, d.nsys = 65536, 256
+    ws = L.tprh_workspace_bytes(ctypes.byref(d))
+    assert ws > 3 * 4 * 65536                                     # scratch for at least one system
+    d.chunk = 3
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * (3 * 65536 + 7 * 256) + 256
+    d.s = 3
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
+    d.s, d.refine, d.coef = 256, 1, _lib.STORE_F16                # refinement needs fp32 coefficients
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
+    # a large system without a workspace is refused before any launch
+    d.refine, d.coef = 0, 0
+    d.a = d.b = d.c = d.d = d.x = 16
+    assert L.tprh_solve_ex(ctypes.byref(d), None) == _lib.ERROR_INVALID_VALUE

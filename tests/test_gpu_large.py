@@ -1,0 +1,168 @@
+"""GPU parity of the general entry point (tprh_solve_ex): systems larger than
+one workgroup (split path), fp16-stored coefficients and the FP64-residual
+refinement step.
+
+Every case runs the exact arithmetic against the oracle bit for bit:
+* split path: the same operations as the one-workgroup path, distributed;
+* fp16 storage: the kernel widens fp16 to fp32 exactly, so it must equal the
+  oracle run on the widened coefficients;
+* refinement: x1 = x0 + A^-1 fl32(d - A x0) with the residual in FP64 in the
+  oracle's operation order (orc_residual_f64) -- identical bits.
+The fast arithmetic is checked against the same references at rel-L2 <= 1e-5.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import tpr_amd
+from tpr_amd import batched as B
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def bits_equal(x, y):
+    x = np.asarray(x, np.float32)
+    y = np.asarray(y, np.float32)
+    same = x.view(np.uint32) == y.view(np.uint32)
+    return bool(np.all(same | (np.isnan(x) & np.isnan(y))))
+
+
+def per_system_rel(x, ref, n):
+    x = x.reshape(-1, n).astype(np.float64)
+    ref = ref.reshape(-1, n).astype(np.float64)
+    return np.linalg.norm(x - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def dev(v, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(v)).to(dtype).cuda()
+
+
+SPLIT_CASES = [(65536, 256, 3, 0), (32768, 128, 4, 3), (16384, 16, 5, 2), (131072, 1024, 2, 1),
+               (8192, 4, 3, 0), (65536, 2048, 2, 0), (4096, 2, 2, 1)]
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nsys,chunk", SPLIT_CASES)
+def test_split_exact_bit_identical(method, n, s, nsys, chunk):
+    say("split exact", method, n, s, nsys, chunk)
+    a, b, c, d, xt = O.gen_batch(31, 0, nsys, n, O.GEN_WEAK)
+    x = torch.empty(nsys * n, device="cuda")
+    B.solve_ex(dev(a), dev(b), dev(c), dev(d), x, n, s, method=method, arith="exact", chunk=chunk)
+    torch.cuda.synchronize()
+    assert bits_equal(x.cpu().numpy(), O.solve_batch(method, a, b, c, d, n, s))
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nsys,chunk", SPLIT_CASES[:4])
+def test_split_fast_within_tolerance(method, n, s, nsys, chunk):
+    say("split fast", method, n, s, nsys, chunk)
+    a, b, c, d, xt = O.gen_batch(32, 0, nsys, n)
+    x = torch.empty(nsys * n, device="cuda")
+    B.solve_ex(dev(a), dev(b), dev(c), dev(d), x, n, s, method=method, arith="fast", chunk=chunk)
+    torch.cuda.synchronize()
+    ref = O.solve_batch(method, a, b, c, d, n, s)
+    assert per_system_rel(x.cpu().numpy(), ref, n).max() <= FAST_TOL
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_split_unit_diagonal_and_host_path(method):
+    """3-argument form through the split path, and the host-pointer entry
+    point (PTPR_CU::ptpr_cu surface) for an n the reference's GPU path cannot
+    solve (n / s > s)."""
+    n, s = 65536, 128
+    say("split unit diag", method)
+    a, b, c, d, _ = O.gen_system(40, 0, n)
+    na, nc, nd = O.normalize(a, b, c, d)
+    ref = (O.ptpr if method == "ptpr" else O.tpr)(na, nc, nd, s)
+    x = torch.empty(n, device="cuda")
+    B.solve_ex(dev(na), None, dev(nc), dev(nd), x, n, s, method=method, arith="exact")
+    torch.cuda.synchronize()
+    assert bits_equal(x.cpu().numpy(), ref)
+    xh = np.zeros(n, np.float32)
+    (tpr_amd.ptpr_cu if method == "ptpr" else tpr_amd.tpr_cu)(na, nc, nd, xh, n, s)
+    assert bits_equal(xh, ref)
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nsys", [(1024, 256, 65), (65536, 256, 2), (2048, 16, 9)])
+def test_fp16_storage_is_exact_on_widened_coefficients(method, n, s, nsys):
+    say("fp16", method, n, s, nsys)
+    a, b, c, d, xt = O.gen_batch(33, 0, nsys, n, O.GEN_WEAK)
+    a16, b16, c16 = (v.astype(np.float16) for v in (a, b, c))
+    x = torch.empty(nsys * n, device="cuda")
+    B.solve_ex(dev(a16, torch.float16), dev(b16, torch.float16), dev(c16, torch.float16), dev(d), x, n, s,
+               method=method, arith="exact")
+    torch.cuda.synchronize()
+    ref = O.solve_batch(method, a16.astype(np.float32), b16.astype(np.float32), c16.astype(np.float32), d, n, s)
+    assert bits_equal(x.cpu().numpy(), ref)
+
+
+def oracle_refine(method, a, b, c, d, x0, n, s):
+    """x0 + A^-1 fl32(d - A x0), residual in FP64 (orc_residual_f64)."""
+    nsys = a.size // n
+    r = np.empty_like(d)
+    for k in range(nsys):
+        sl = slice(k * n, (k + 1) * n)
+        r[sl] = O.residual_f64(a[sl], b[sl], c[sl], d[sl], x0[sl].astype(np.float64)).astype(np.float32)
+    dx = O.solve_batch(method, a, b, c, r, n, s)
+    return (x0 + dx).astype(np.float32)
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nsys", [(1024, 256, 33), (65536, 256, 2), (512, 16, 7)])
+def test_refinement_step_bit_identical(method, n, s, nsys):
+    say("refine", method, n, s, nsys)
+    a, b, c, d, xt = O.gen_batch(34, 0, nsys, n, O.GEN_WEAK)
+    da, db, dc, dd = dev(a), dev(b), dev(c), dev(d)
+    x = torch.empty(nsys * n, device="cuda")
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, arith="exact")
+    torch.cuda.synchronize()
+    x0 = x.cpu().numpy()
+    assert bits_equal(x0, O.solve_batch(method, a, b, c, d, n, s))
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, arith="exact", refine=True)
+    torch.cuda.synchronize()
+    x1 = x.cpu().numpy()
+    assert bits_equal(x1, oracle_refine(method, a, b, c, d, x0, n, s))
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_refinement_reaches_fp64_accuracy(method):
+    """Weakly dominant systems (config 3 generator, smaller n): the fp16-stored
+    solve is the least accurate, fp32 next; refinement steps against the fp32
+    system bring both to the FP64 Thomas solution within fp32 rounding."""
+    n, s, nsys = 16384, 256, 8
+    say("refine accuracy", method)
+    a, b, c, d, xt = O.gen_batch(35, 0, nsys, n, O.GEN_WEAK)
+    xf = O.thomas_f64_batch(a, b, c, d, n)
+    da, db, dc, dd = dev(a), dev(b), dev(c), dev(d)
+    x = torch.empty(nsys * n, device="cuda")
+    errs = {}
+    B.solve_ex(dev(a.astype(np.float16), torch.float16), dev(b.astype(np.float16), torch.float16),
+               dev(c.astype(np.float16), torch.float16), dd, x, n, s, method=method)
+    torch.cuda.synchronize()
+    errs["f16"] = per_system_rel(x.cpu().numpy(), xf, n).max()
+    for step in (1, 2, 3):
+        B.solve_ex(da, db, dc, dd, x, n, s, method=method, refine=True)
+        torch.cuda.synchronize()
+        errs["f16+%d" % step] = per_system_rel(x.cpu().numpy(), xf, n).max()
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method)
+    torch.cuda.synchronize()
+    errs["f32"] = per_system_rel(x.cpu().numpy(), xf, n).max()
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, refine=True)
+    torch.cuda.synchronize()
+    errs["f32+1"] = per_system_rel(x.cpu().numpy(), xf, n).max()
+    say(errs)
+    assert errs["f32+1"] <= errs["f32"]
+    assert errs["f32+1"] <= 2e-7
+    assert errs["f16+3"] <= 2e-7
+    assert errs["f16"] > errs["f32"]

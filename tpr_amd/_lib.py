@@ -47,6 +47,28 @@ SIGNATURES = {
     "tprh_generate": (_i, [_i, _u64, _u64, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
+
+
+class SolveDesc(ctypes.Structure):
+    """tprh_solve_desc of include/tpr_hip.h."""
+    _fields_ = [
+        ("method", ctypes.c_int), ("arith", ctypes.c_int),
+        ("n", ctypes.c_int), ("s", ctypes.c_int), ("nsys", ctypes.c_int),
+        ("coef", ctypes.c_int),
+        ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+        ("d", ctypes.c_void_p), ("x", ctypes.c_void_p),
+        ("refine", ctypes.c_int), ("E", ctypes.c_int), ("L", ctypes.c_int), ("chunk", ctypes.c_int),
+        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+SIGNATURES.update({
+    "tprh_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(SolveDesc)]),
+    "tprh_solve_ex": (_i, [ctypes.POINTER(SolveDesc), _vp]),
+})
+
+STORE_F32, STORE_F16 = 0, 1
+
 _lib = None
 
 

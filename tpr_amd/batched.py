@@ -52,6 +52,67 @@ def solve_batched(a, b, c, d, x, n, s, method="ptpr", arith="fast", shape=None, 
     return x
 
 
+def _dptr_any(t, name, numel, dtypes):
+    import torch
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous() or t.dtype not in dtypes:
+        raise TypeError("%s must be a contiguous GPU tensor of %s" % (name, dtypes))
+    if t.numel() < numel:
+        raise ValueError("%s has %d elements, need %d" % (name, t.numel(), numel))
+    return t.data_ptr()
+
+
+def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0):
+    """tprh_solve_desc for torch tensors; a, b, c float32 or float16 (same dtype)."""
+    import torch
+    nsys = d.numel() // n
+    tot = nsys * n
+    coef16 = a.dtype == torch.float16
+    cd = (torch.float16,) if coef16 else (torch.float32,)
+    desc = _lib.SolveDesc()
+    desc.method = METHODS[method]
+    desc.arith = ARITH[arith]
+    desc.n, desc.s, desc.nsys = int(n), int(s), int(nsys)
+    desc.coef = _lib.STORE_F16 if coef16 else _lib.STORE_F32
+    desc.a = _dptr_any(a, "a", tot, cd)
+    desc.b = _dptr_any(b, "b", tot, cd)
+    desc.c = _dptr_any(c, "c", tot, cd)
+    desc.d = _dptr_any(d, "d", tot, (torch.float32,))
+    desc.x = _dptr_any(x, "x", tot, (torch.float32,))
+    desc.refine = 1 if refine else 0
+    desc.E, desc.L = shape if shape is not None else (0, 0)
+    desc.chunk = int(chunk)
+    return desc
+
+
+def workspace_bytes(desc):
+    v = _lib.lib().tprh_workspace_bytes(ctypes.byref(desc))
+    if v == ctypes.c_size_t(-1).value:
+        _lib.check(_lib.ERROR_INVALID_VALUE, "tprh_workspace_bytes")
+    return v
+
+
+def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
+             workspace=None, stream=None):
+    """The general entry point (tprh_solve_ex): large systems (through a
+    workspace, allocated here from torch if not given), fp16 coefficient
+    storage (a, b, c as float16 tensors), one refinement step (refine=True:
+    x += A^-1 (d - A x) with an FP64 residual)."""
+    import torch
+    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk)
+    need = workspace_bytes(desc)
+    if need:
+        if workspace is None:
+            workspace = torch.empty(need, dtype=torch.uint8, device=d.device)
+        if workspace.numel() * workspace.element_size() < need:
+            raise ValueError("workspace smaller than %d bytes" % need)
+        desc.workspace = workspace.data_ptr()
+        desc.workspace_bytes = workspace.numel() * workspace.element_size()
+    _lib.call("tprh_solve_ex", ctypes.byref(desc), _stream(stream))
+    return x
+
+
 def shape_supported(E, L):
     return bool(_lib.lib().tprh_shape_supported(int(E), int(L)))
 

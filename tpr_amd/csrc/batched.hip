@@ -14,39 +14,74 @@ namespace tprh {
 
 // One workgroup holds sys_per_block systems side by side (or one system of up
 // to BLOCK lanes).  HAS_B: the diagonal is given and divided out in registers
-// (false: unit diagonal, b is not read).
-template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B>
-__global__ __launch_bounds__(BLOCK) void batched_kernel(const float* __restrict__ a, const float* __restrict__ b,
-                                                       const float* __restrict__ c, const float* __restrict__ d,
+// (false: unit diagonal, b is not read).  CT: storage type of a, b, c (float
+// or _Float16).  REFINE: one iterative-refinement step, x += A^-1 (d - A x)
+// with the residual accumulated in FP64.
+template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B, typename CT, bool REFINE>
+__global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
+                                                       const CT* __restrict__ c, const float* __restrict__ d,
                                                        float* __restrict__ x, int n, int m, int nsys,
                                                        int lg_lanes) {
     extern __shared__ float lds[];
-    const int sl = threadIdx.x >> lg_lanes;                 // system slot in the workgroup
-    const int t = threadIdx.x & ((1 << lg_lanes) - 1);      // lane inside the system
+    const int sl = threadIdx.x >> lg_lanes;             // system slot in the workgroup
+    const int t = threadIdx.x & ((1 << lg_lanes) - 1);  // lane inside the system
     float* my = lds + sl * lds_words_per_system(m);
     const int sys = blockIdx.x * (blockDim.x >> lg_lanes) + sl;
     const bool valid = sys < nsys;
-    const size_t off = (valid ? (size_t)sys * (size_t)n : 0) + (size_t)t * E;
+    const size_t x0 = valid ? (size_t)sys * (size_t)n : 0;
+    const size_t off = x0 + (size_t)t * E;
     RawPart<E> raw;
-    load_raw<E>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    float xo[E];
+    if constexpr (REFINE) residual_fp64<E, HAS_B>(x, x0, off, n, raw, xo);
+    float A[E], C[E], R[E], X[E];
+    normalise<E, MODE>(raw, HAS_B, A, C, R);
     if constexpr (METHOD == 0) {
-        ptpr_solve_lanes<E, L, MODE>(raw, HAS_B, x, off, m, t, valid, my);
+        ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
     } else {
-        tpr_solve_lanes<E, L, MODE>(raw, HAS_B, x, off, m, t, valid, my);
+        tpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
     }
+    if constexpr (REFINE) {
+#pragma unroll
+        for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
+    }
+    if (valid) store_e<E>(x + off, X);
 }
 
 namespace {
 
-template <int METHOD, int E, int L, int MODE, int BLOCK>
+template <int METHOD, int E, int L, int MODE, int BLOCK, typename CT, bool REFINE>
 void launch_b(const BatchArgs& p, int grid, int block, size_t shmem, int lg, hipStream_t stream) {
+    const CT* a = static_cast<const CT*>(p.a);
+    const CT* b = static_cast<const CT*>(p.b);
+    const CT* c = static_cast<const CT*>(p.c);
     if (p.b != nullptr) {
-        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, true>), dim3(grid), dim3(block), shmem, stream,
-                           p.a, p.b, p.c, p.d, p.x, p.n, p.m, p.nsys, lg);
+        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, true, CT, REFINE>), dim3(grid), dim3(block),
+                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg);
     } else {
-        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, false>), dim3(grid), dim3(block), shmem, stream,
-                           p.a, p.b, p.c, p.d, p.x, p.n, p.m, p.nsys, lg);
+        hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, false, CT, REFINE>), dim3(grid), dim3(block),
+                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg);
     }
+}
+
+template <int METHOD, int E, int L, int MODE, int BLOCK>
+hipError_t launch_variant(const BatchArgs& p, int grid, int block, size_t shmem, int lg, hipStream_t stream) {
+    if (!p.f16 && !p.refine) {
+        launch_b<METHOD, E, L, MODE, BLOCK, float, false>(p, grid, block, shmem, lg, stream);
+        return hipSuccess;
+    }
+    // the storage and refinement variants exist for the default shapes only
+    if constexpr (is_default_shape(E, L)) {
+        if (p.f16 && !p.refine) {
+            launch_b<METHOD, E, L, MODE, BLOCK, _Float16, false>(p, grid, block, shmem, lg, stream);
+            return hipSuccess;
+        }
+        if (!p.f16 && p.refine) {
+            launch_b<METHOD, E, L, MODE, BLOCK, float, true>(p, grid, block, shmem, lg, stream);
+            return hipSuccess;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 template <int METHOD, int E, int L, int MODE>
@@ -57,11 +92,9 @@ hipError_t launch_el(const BatchArgs& p, hipStream_t stream) {
     const int spb = block / T;
     const int grid = (p.nsys + spb - 1) / spb;
     const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(p.m);
-    if (block == 64) {
-        launch_b<METHOD, E, L, MODE, 64>(p, grid, block, shmem, lg, stream);
-    } else {
-        launch_b<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
-    }
+    hipError_t e = block == 64 ? launch_variant<METHOD, E, L, MODE, 64>(p, grid, block, shmem, lg, stream)
+                               : launch_variant<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

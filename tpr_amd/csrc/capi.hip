@@ -13,6 +13,7 @@
 #include "../../include/tpr_hip.h"
 #include "launch.hpp"
 #include "launch_gen.hpp"
+#include "launch_split.hpp"
 
 namespace {
 
@@ -41,7 +42,8 @@ bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 // Checks (n, s) for PTPR/TPR and picks the slice shape.  The reference
 // requires s and m = n / s to be powers of two (src/ptpr.cpp:51-52,
 // src/pcr.cpp:16) and silently misbehaves otherwise; here they are errors.
-tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape) {
+// *split is set when a system needs more lanes than one workgroup has.
+tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape, bool allow_split, bool* split) {
     if (n <= 0 || s <= 0 || nsys < 0) return fail(TPRH_ERROR_INVALID_VALUE, "n, s must be positive and nsys >= 0");
     if (!is_pow2(s) || s < 2) return fail(TPRH_ERROR_INVALID_VALUE, "s must be a power of two >= 2");
     if (n % s != 0) return fail(TPRH_ERROR_INVALID_VALUE, "n must be a multiple of s");
@@ -53,9 +55,89 @@ tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape
     } else if (!tprh::default_shape(s, E, L)) {
         return fail(TPRH_ERROR_NOT_SUPPORTED, "slice size s above 2048 is not supported");
     }
-    if ((long long)m * *L > 1024)
-        return fail(TPRH_ERROR_NOT_SUPPORTED,
-                    "n / s * lanes-per-slice exceeds one workgroup (1024 lanes); use a larger s");
+    *split = (long long)m * *L > 1024;
+    if (*split) {
+        if (!allow_split)
+            return fail(TPRH_ERROR_NOT_SUPPORTED,
+                        "n / s * lanes-per-slice exceeds one workgroup (1024 lanes): use tprh_solve_ex with a "
+                        "workspace, or a larger s");
+        if (m > tprh::kSplitMaxM) return fail(TPRH_ERROR_NOT_SUPPORTED, "more than 2048 slices per system");
+        if (!tprh::is_default_shape(*E, *L))
+            return fail(TPRH_ERROR_NOT_SUPPORTED, "the multi-workgroup path uses the default slice shapes only");
+    }
+    return TPRH_SUCCESS;
+}
+
+// Systems per round of the multi-workgroup path: inputs (16 B/eq) and
+// scratch (12 B/eq) of a round within 128 MiB, half the Infinity Cache.
+int default_chunk(int n, int nsys) {
+    long long c = (128LL << 20) / (28LL * n);
+    if (c < 1) c = 1;
+    if (c > nsys) c = nsys;
+    return (int)c;
+}
+
+struct Plan {
+    int E = 0, L = 0, m = 0, chunk = 0;
+    bool split = false;
+    size_t ws = 0;
+};
+
+tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
+    if (d.method != TPRH_PTPR && d.method != TPRH_TPR)
+        return fail(TPRH_ERROR_INVALID_VALUE, "method must be TPRH_PTPR or TPRH_TPR");
+    if (d.arith != TPRH_ARITH_FAST && d.arith != TPRH_ARITH_EXACT) return fail(TPRH_ERROR_INVALID_VALUE, "bad arith");
+    if (d.coef != TPRH_STORE_F32 && d.coef != TPRH_STORE_F16) return fail(TPRH_ERROR_INVALID_VALUE, "bad storage");
+    if (d.refine != 0 && d.refine != 1) return fail(TPRH_ERROR_INVALID_VALUE, "refine must be 0 or 1");
+    if (d.refine && d.coef != TPRH_STORE_F32)
+        return fail(TPRH_ERROR_INVALID_VALUE, "refinement needs fp32 coefficients for the residual");
+    const bool explicit_shape = d.E != 0 || d.L != 0;
+    pl->E = d.E;
+    pl->L = d.L;
+    tprh_status st = check_ns(d.n, d.s, d.nsys, &pl->E, &pl->L, explicit_shape, true, &pl->split);
+    if (st != TPRH_SUCCESS) return st;
+    if ((d.coef == TPRH_STORE_F16 || d.refine) && !tprh::is_default_shape(pl->E, pl->L))
+        return fail(TPRH_ERROR_NOT_SUPPORTED, "fp16 storage and refinement use the default slice shapes only");
+    if (d.chunk < 0) return fail(TPRH_ERROR_INVALID_VALUE, "chunk must be >= 0");
+    pl->m = d.n / d.s;
+    pl->chunk = d.chunk > 0 ? (d.chunk < d.nsys ? d.chunk : d.nsys) : default_chunk(d.n, d.nsys);
+    if (pl->chunk < 1) pl->chunk = 1;
+    pl->ws = pl->split ? tprh::split_workspace_bytes(d.n, pl->m, pl->chunk) : 0;
+    return TPRH_SUCCESS;
+}
+
+tprh_status solve_desc(const tprh_solve_desc& d, hipStream_t stream) {
+    Plan pl;
+    tprh_status st = plan_desc(d, &pl);
+    if (st != TPRH_SUCCESS) return st;
+    if (d.nsys == 0) return TPRH_SUCCESS;
+    if (!d.a || !d.c || !d.d || !d.x) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
+    const int method = d.method == TPRH_PTPR ? 0 : 1;
+    const int mode = d.arith == TPRH_ARITH_EXACT ? 1 : 0;
+    hipError_t e;
+    if (!pl.split) {
+        tprh::BatchArgs p{d.a, d.b, d.c, d.d, d.x, d.n, pl.m, d.nsys};
+        p.f16 = d.coef == TPRH_STORE_F16;
+        p.refine = d.refine != 0;
+        e = tprh::launch_batched(method, mode, p, pl.E, pl.L, stream);
+    } else {
+        if (!d.workspace || d.workspace_bytes < pl.ws)
+            return fail(TPRH_ERROR_INVALID_VALUE, "the multi-workgroup path needs tprh_workspace_bytes() of workspace");
+        tprh::SplitArgs p{};
+        p.a = d.a;
+        p.b = d.b;
+        p.c = d.c;
+        p.d = d.d;
+        p.x = d.x;
+        p.n = d.n;
+        p.nsys = d.nsys;
+        p.chunk = pl.chunk;
+        p.f16 = d.coef == TPRH_STORE_F16;
+        p.refine = d.refine != 0;
+        tprh::split_carve(d.workspace, d.n, pl.m, pl.chunk, &p);
+        e = tprh::launch_split(method, mode, p, pl.E, pl.L, stream);
+    }
+    if (e != hipSuccess) return fail_hip(e, "solver kernel launch");
     return TPRH_SUCCESS;
 }
 
@@ -64,16 +146,24 @@ tprh_status batched(tprh_method method, tprh_arith arith, const float* a, const 
                     void* stream) {
     if (method != TPRH_PTPR && method != TPRH_TPR)
         return fail(TPRH_ERROR_INVALID_VALUE, "batched solve: method must be TPRH_PTPR or TPRH_TPR");
-    if (arith != TPRH_ARITH_FAST && arith != TPRH_ARITH_EXACT) return fail(TPRH_ERROR_INVALID_VALUE, "bad arith");
-    tprh_status st = check_ns(n, s, nsys, &E, &L, explicit_shape);
+    bool split = false;
+    tprh_status st = check_ns(n, s, nsys, &E, &L, explicit_shape, false, &split);
     if (st != TPRH_SUCCESS) return st;
-    if (nsys == 0) return TPRH_SUCCESS;
-    if (!a || !c || !d || !x) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
-    tprh::BatchArgs p{a, b, c, d, x, n, n / s, nsys};
-    hipError_t e = tprh::launch_batched(method == TPRH_PTPR ? 0 : 1, arith == TPRH_ARITH_EXACT ? 1 : 0, p, E, L,
-                                        (hipStream_t)stream);
-    if (e != hipSuccess) return fail_hip(e, "batched kernel launch");
-    return TPRH_SUCCESS;
+    tprh_solve_desc desc{};
+    desc.method = method;
+    desc.arith = arith;
+    desc.n = n;
+    desc.s = s;
+    desc.nsys = nsys;
+    desc.coef = TPRH_STORE_F32;
+    desc.a = a;
+    desc.b = b;
+    desc.c = c;
+    desc.d = d;
+    desc.x = x;
+    desc.E = E;
+    desc.L = L;
+    return solve_desc(desc, (hipStream_t)stream);
 }
 
 // Device buffers for the host-pointer entry points, grown on demand and kept
@@ -90,9 +180,16 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
                             const float* c, const float* rhs, float* x, int n, int s) {
     if (!a || !c || !rhs || !x) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
     if (n <= 0) return fail(TPRH_ERROR_INVALID_VALUE, "n must be positive");
-    int E = 0, L = 0;
+    tprh_solve_desc desc{};
+    Plan pl;
     if (method == TPRH_PTPR || method == TPRH_TPR) {
-        tprh_status st = check_ns(n, s, 1, &E, &L, false);
+        desc.method = method;
+        desc.arith = arith;
+        desc.n = n;
+        desc.s = s;
+        desc.nsys = 1;
+        desc.coef = TPRH_STORE_F32;
+        tprh_status st = plan_desc(desc, &pl);
         if (st != TPRH_SUCCESS) return st;
     } else if (method == TPRH_PCR || method == TPRH_CR) {
         if (!is_pow2(n)) return fail(TPRH_ERROR_INVALID_VALUE, "PCR / CR: n must be a power of two");
@@ -101,7 +198,7 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
         return fail(TPRH_ERROR_INVALID_VALUE, "unknown method");
     }
     std::lock_guard<std::mutex> lock(g_host_bufs.mu);
-    const size_t need = 5 * (size_t)n;
+    const size_t need = 5 * (size_t)n + (pl.ws + sizeof(float) - 1) / sizeof(float);
     if (g_host_bufs.cap < need) {
         if (g_host_bufs.dev) (void)hipFree(g_host_bufs.dev);
         g_host_bufs.dev = nullptr;
@@ -114,6 +211,7 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     float* dr = dc + n;
     float* dx = dr + n;
     float* db = dx + n;
+    float* ws = db + n;
     const size_t bytes = sizeof(float) * (size_t)n;
     TPRH_HIP(hipMemcpy(da, a, bytes, hipMemcpyHostToDevice));
     TPRH_HIP(hipMemcpy(dc, c, bytes, hipMemcpyHostToDevice));
@@ -125,23 +223,31 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
         TPRH_HIP(hipEventCreate(&ev[1]));
         TPRH_HIP(hipEventRecord(ev[0], nullptr));
     }
-    hipError_t le;
+    tprh_status st = TPRH_SUCCESS;
+    hipError_t le = hipSuccess;
     if (method == TPRH_PTPR || method == TPRH_TPR) {
-        tprh::BatchArgs p{da, diag ? db : nullptr, dc, dr, dx, n, n / s, 1};
-        le = tprh::launch_batched(method == TPRH_PTPR ? 0 : 1, arith == TPRH_ARITH_EXACT ? 1 : 0, p, E, L, nullptr);
+        desc.a = da;
+        desc.b = diag ? db : nullptr;
+        desc.c = dc;
+        desc.d = dr;
+        desc.x = dx;
+        desc.workspace = ws;
+        desc.workspace_bytes = pl.ws;
+        st = solve_desc(desc, nullptr);
     } else {
         le = tprh::launch_single(method, da, diag ? db : nullptr, dc, dr, dx, n, 1, nullptr);
     }
     g_last_kernel_ms = -1.0f;
     if (g_timing) {
         hipError_t e1 = hipEventRecord(ev[1], nullptr);
-        if (le == hipSuccess && e1 == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess) {
+        if (st == TPRH_SUCCESS && le == hipSuccess && e1 == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess) {
             float ms = -1.0f;
             if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) g_last_kernel_ms = ms;
         }
         (void)hipEventDestroy(ev[0]);
         (void)hipEventDestroy(ev[1]);
     }
+    if (st != TPRH_SUCCESS) return st;
     TPRH_HIP(le);
     TPRH_HIP(hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost));
     return TPRH_SUCCESS;
@@ -213,7 +319,8 @@ tprh_status tprh_solver_create(tprh_solver** out, tprh_method method, int n, int
     *out = nullptr;
     if (method == TPRH_PTPR || method == TPRH_TPR) {
         int E = 0, L = 0;
-        tprh_status st = check_ns(n, s, 1, &E, &L, false);
+        bool split = false;
+        tprh_status st = check_ns(n, s, 1, &E, &L, false, true, &split);
         if (st != TPRH_SUCCESS) return st;
     } else if (method == TPRH_PCR || method == TPRH_CR) {
         if (!is_pow2(n)) return fail(TPRH_ERROR_INVALID_VALUE, "PCR / CR: n must be a power of two");
@@ -304,6 +411,18 @@ tprh_status tprh_solve_batched_shape(tprh_method method, tprh_arith arith, const
 }
 
 int tprh_shape_supported(int E, int L) { return tprh::shape_compiled(E, L) ? 1 : 0; }
+
+size_t tprh_workspace_bytes(const tprh_solve_desc* desc) {
+    if (!desc) return (size_t)-1;
+    Plan pl;
+    if (plan_desc(*desc, &pl) != TPRH_SUCCESS) return (size_t)-1;
+    return pl.ws;
+}
+
+tprh_status tprh_solve_ex(const tprh_solve_desc* desc, void* stream) {
+    if (!desc) return fail(TPRH_ERROR_INVALID_VALUE, "null descriptor");
+    return solve_desc(*desc, (hipStream_t)stream);
+}
 
 tprh_status tprh_generate(tprh_gen kind, uint64_t seed, uint64_t sys0, int nsys, int n, float* a, float* b, float* c,
                           float* d, float* xt, void* stream) {

@@ -51,15 +51,22 @@ inline bool default_shape(int s, int* E, int* L) {
     }
 }
 
+constexpr bool is_default_shape(int E, int L) {
+    return (E == 1 && (L == 2 || L == 4 || L == 8 || L == 16)) || (L == 16 && (E == 2 || E == 4 || E == 8 || E == 16)) ||
+           (E == 16 && (L == 32 || L == 64)) || (E == 32 && L == 64);
+}
+
 struct BatchArgs {
-    const float* a;
-    const float* b;  // may be null: unit diagonal
-    const float* c;
+    const void* a;  // float, or _Float16 when f16
+    const void* b;  // may be null: unit diagonal
+    const void* c;
     const float* d;
     float* x;
     int n;     // equations per system
     int m;     // slices per system (n / s)
     int nsys;  // systems
+    bool f16 = false;     // a, b, c stored as fp16
+    bool refine = false;  // x += A^-1 (d - A x), residual in FP64
 };
 
 // method 0 = PTPR, 1 = TPR; mode 0 = fast, 1 = exact.

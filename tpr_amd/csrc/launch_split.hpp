@@ -1,0 +1,62 @@
+// launch_split.hpp -- host-side interface of the multi-workgroup (split) path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace tprh {
+
+// Slice shapes compiled for the split path (the default shape of every s).
+#define TPRH_SPLIT_SHAPES(X) \
+    X(1, 2)                  \
+    X(1, 4)                  \
+    X(1, 8)                  \
+    X(1, 16)                 \
+    X(2, 16)                 \
+    X(4, 16)                 \
+    X(8, 16)                 \
+    X(16, 16)                \
+    X(16, 32)                \
+    X(16, 64)                \
+    X(32, 64)
+
+// Largest reduced system (slices per system) the split path solves in one
+// workgroup's LDS: 10 rows of m floats.
+constexpr int kSplitMaxM = 2048;
+
+struct SplitArgs {
+    const void* a;  // float or _Float16 (f16)
+    const void* b;  // may be null: unit diagonal
+    const void* c;
+    const float* d;
+    float* x;
+    int n, nsys, chunk;
+    bool f16, refine;
+    // workspace, for one round of `chunk` systems
+    float *sA, *sC, *sR;  // chunk * n each
+    float* rows;          // chunk * m * 6
+    float* xed;           // chunk * m
+};
+
+// Bytes of workspace for rounds of `chunk` systems of n equations, m slices.
+inline size_t split_workspace_bytes(int n, int m, int chunk) {
+    return sizeof(float) * (size_t)chunk * (3 * (size_t)n + 7 * (size_t)m) + 256;
+}
+
+// Carves the workspace (256-byte aligned) into the SplitArgs pointers.
+inline void split_carve(void* ws, int n, int m, int chunk, SplitArgs* p) {
+    size_t addr = ((size_t)ws + 255) & ~(size_t)255;
+    float* f = reinterpret_cast<float*>(addr);
+    p->sA = f;
+    f += (size_t)chunk * n;
+    p->sC = f;
+    f += (size_t)chunk * n;
+    p->sR = f;
+    f += (size_t)chunk * n;
+    p->rows = f;
+    f += (size_t)chunk * m * 6;
+    p->xed = f;
+}
+
+hipError_t launch_split(int method, int mode, const SplitArgs& p, int E, int L, hipStream_t stream);
+
+}  // namespace tprh

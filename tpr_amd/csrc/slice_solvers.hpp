@@ -86,6 +86,62 @@ __device__ __forceinline__ void load_raw(const float* __restrict__ a, const floa
     if (b != nullptr) load_e<E>(b + off, p.b);
 }
 
+// E consecutive fp16 values, widened to fp32 (exact).
+template <int E>
+__device__ __forceinline__ void load_e(const _Float16* __restrict__ p, float (&v)[E]) {
+    if constexpr (E % 8 == 0) {
+#pragma unroll
+        for (int i = 0; i < E / 8; i++) {
+            uint4 t = reinterpret_cast<const uint4*>(p)[i];
+            const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                v[8 * i + 2 * k] = (float)__builtin_bit_cast(_Float16, (uint16_t)(w[k] & 0xFFFF));
+                v[8 * i + 2 * k + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(w[k] >> 16));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = (float)p[i];
+    }
+}
+
+// Coefficients stored as CT (float or _Float16), right-hand side as float.
+template <int E, typename CT>
+__device__ __forceinline__ void load_raw_t(const CT* __restrict__ a, const CT* __restrict__ b,
+                                           const CT* __restrict__ c, const float* __restrict__ d, size_t off,
+                                           RawPart<E>& p) {
+    load_e<E>(a + off, p.a);
+    load_e<E>(c + off, p.c);
+    load_e<E>(d + off, p.d);
+    if (b != nullptr) load_e<E>(b + off, p.b);
+}
+
+// Iterative refinement: replaces p.d by the residual r = d - A x of the
+// system at the current x, accumulated in FP64 (products of two floats are
+// exact in double, so the only roundings are the three additions and the
+// final conversion).  x0 is the system's first equation (global index), i0
+// the lane's first equation, n the system size; the lane reads x[i0-1 ..
+// i0+E] (neighbours outside the system count as 0, as a[0] and c[n-1] are
+// outside the matrix).  HAS_B == false: unit diagonal.
+template <int E, bool HAS_B>
+__device__ __forceinline__ void residual_fp64(const float* __restrict__ x, size_t x0, size_t i0, int n,
+                                              RawPart<E>& p, float (&xo)[E]) {
+    load_e<E>(x + i0, xo);
+    const float xl = (i0 > x0) ? x[i0 - 1] : 0.0f;
+    const float xr = (i0 + E < x0 + (size_t)n) ? x[i0 + E] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const double xm = (k == 0) ? (double)xl : (double)xo[k - 1];
+        const double xp = (k == E - 1) ? (double)xr : (double)xo[k + 1];
+        const double bk = HAS_B ? (double)p.b[k] : 1.0;
+        double acc = (double)p.d[k] - bk * (double)xo[k];
+        acc -= (double)p.a[k] * xm;
+        acc -= (double)p.c[k] * xp;
+        p.d[k] = (float)acc;
+    }
+}
+
 // Divides by the diagonal as the 4-argument setter does (src/ptpr.hpp:80-90);
 // has_b == false means a unit diagonal (the reference's GPU entry points).
 template <int E, int MODE>
@@ -441,18 +497,12 @@ __device__ __forceinline__ SliceEnd slice_end(const float* lds, int m, int j) {
 }
 
 // ---------------------------------------------------------------------------
-// Whole PTPR solve of one system by its lanes, from the raw loads of the
-// lane's E equations at element offset `off`.  sys_valid == false lanes run
-// the same instruction stream on zeros (every lane must take part in the
-// cross-lane exchanges and the barriers) and store nothing.
-template <int E, int L, int MODE>
-__device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
-                                                 size_t off, int m, int t, bool sys_valid, float* lds) {
-    const int l = t % L;
-    const int j = t / L;
-    float A[E], C[E], R[E];
-    normalise<E, MODE>(raw, has_b, A, C, R);
+// Stage entry points shared by the batched (one workgroup per system) and the
+// split (several workgroups per system) kernels.
 
+// PTPR stage 1 on the lane's registers (src/ptpr.cpp:130-190).
+template <int E, int L, int MODE>
+__device__ __forceinline__ void ptpr_stage1(float (&A)[E], float (&C)[E], float (&R)[E], int l) {
     if constexpr (MODE == kFast && L == 16) {
 #pragma unroll
         for (int i = 0; i < E; i++) {
@@ -468,13 +518,14 @@ __device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has
     } else {
         ptpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
     }
+}
 
-    const float* xr = reduced_solve<0>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
-
-    // stage 3, src/ptpr.cpp:263-287 (E_st keeps its pre-update values in A/C/R)
-    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
-    const float xe = xr[j];
-    SliceEnd se = slice_end(lds, m, j);
+// PTPR stage 3 (src/ptpr.cpp:263-287): x_i = r_i - a_i x[st-1] - c_i key with
+// key = (r_ed - a_ed x[st-1] - x_ed) / c_ed (0 if c_ed == 0); the slice's last
+// equation takes x_ed.  A/C/R hold the stage-1 values (E_st before its update).
+template <int E, int L, int MODE>
+__device__ __forceinline__ void ptpr_stage3(const float (&A)[E], const float (&C)[E], const float (&R)[E],
+                                            float xprev, float xe, SliceEnd se, int l, float (&X)[E]) {
     float key;
     if constexpr (MODE == kExact) {
         key = 1.0f / se.c * (se.r - se.a * xprev - xe);
@@ -482,7 +533,6 @@ __device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has
         key = fast_rcp(se.c) * (__builtin_fmaf(-se.a, xprev, se.r) - xe);
     }
     if (se.c == 0.0f) key = 0.0f;
-    float X[E];
 #pragma unroll
     for (int i = 0; i < E; i++) {
         if constexpr (MODE == kExact) {
@@ -492,17 +542,41 @@ __device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has
         }
     }
     if (l == L - 1) X[E - 1] = xe;
-    if (sys_valid) store_e<E>(x + off, X);
 }
 
-// Whole TPR solve of one system by its lanes.
+// TPR stage 3 (src/tpr.cpp:277-308) from the stage-3 coefficients (A/C/R:
+// stage-1 values, Ao/Co/Ro: originals; even offsets use the originals).
 template <int E, int L, int MODE>
-__device__ __forceinline__ void tpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
-                                                size_t off, int m, int t, bool sys_valid, float* lds) {
+__device__ __forceinline__ void tpr_stage3(const float (&A)[E], const float (&C)[E], const float (&R)[E],
+                                           const float (&Ao)[E], const float (&Co)[E], const float (&Ro)[E],
+                                           float xprev, float xe, int l, float (&X)[E]) {
+#pragma unroll
+    for (int i = 0; i < E; i++) X[i] = 0.0f;
+    X[E - 1] = (l == L - 1) ? xe : 0.0f;
+    tpr_stage3_level<E, L, MODE, (E * L) / 2>(A, C, R, Ao, Co, Ro, X, xprev, l);
+}
+
+// ---------------------------------------------------------------------------
+// Whole PTPR solve of one system by its lanes, from the raw loads of the
+// lane's E equations at element offset `off`.  sys_valid == false lanes run
+// the same instruction stream on zeros (every lane must take part in the
+// cross-lane exchanges and the barriers) and store nothing.
+template <int E, int L, int MODE>
+__device__ __forceinline__ void ptpr_solve_regs(float (&A)[E], float (&C)[E], float (&R)[E], int m, int t,
+                                                float* lds, float (&X)[E]) {
     const int l = t % L;
     const int j = t / L;
-    float A[E], C[E], R[E];
-    normalise<E, MODE>(raw, has_b, A, C, R);
+    ptpr_stage1<E, L, MODE>(A, C, R, l);
+    const float* xr = reduced_solve<0>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+    ptpr_stage3<E, L, MODE>(A, C, R, xprev, xr[j], slice_end(lds, m, j), l, X);
+}
+
+template <int E, int L, int MODE>
+__device__ __forceinline__ void tpr_solve_regs(float (&A)[E], float (&C)[E], float (&R)[E], int m, int t,
+                                               float* lds, float (&X)[E]) {
+    const int l = t % L;
+    const int j = t / L;
     float Ao[E], Co[E], Ro[E];
 #pragma unroll
     for (int i = 0; i < E; i++) {
@@ -510,17 +584,27 @@ __device__ __forceinline__ void tpr_solve_lanes(const RawPart<E>& raw, bool has_
         Co[i] = C[i];
         Ro[i] = R[i];
     }
-
     tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
-
     const float* xr = reduced_solve<1>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
-
     const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
-    float X[E];
-#pragma unroll
-    for (int i = 0; i < E; i++) X[i] = 0.0f;
-    X[E - 1] = (l == L - 1) ? xr[j] : 0.0f;
-    tpr_stage3_level<E, L, MODE, (E * L) / 2>(A, C, R, Ao, Co, Ro, X, xprev, l);
+    tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xr[j], l, X);
+}
+
+template <int E, int L, int MODE>
+__device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
+                                                 size_t off, int m, int t, bool sys_valid, float* lds) {
+    float A[E], C[E], R[E], X[E];
+    normalise<E, MODE>(raw, has_b, A, C, R);
+    ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, lds, X);
+    if (sys_valid) store_e<E>(x + off, X);
+}
+
+template <int E, int L, int MODE>
+__device__ __forceinline__ void tpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
+                                                size_t off, int m, int t, bool sys_valid, float* lds) {
+    float A[E], C[E], R[E], X[E];
+    normalise<E, MODE>(raw, has_b, A, C, R);
+    tpr_solve_regs<E, L, MODE>(A, C, R, m, t, lds, X);
     if (sys_valid) store_e<E>(x + off, X);
 }
 
