@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
+    ap.add_argument("--chunk", type=int, default=0, help="config 3: systems per round of the split path")
+    ap.add_argument("--config", type=int, default=1, choices=[1, 3],
+                    help="1: the headline batch (default); 3: 256 x N=65536 weakly dominant, precision variants")
     return ap.parse_args()
 
 
@@ -152,12 +155,112 @@ def cpu_leg(args, host, x_gpu):
     return base, acc
 
 
+def config3(args):
+    """256 weakly dominant systems x N=65536, S=256 (m = 256 slices: the
+    multi-workgroup path).  Variants: fp32 PTPR, fp16-stored coefficients,
+    each followed by 0..2 FP64-residual refinement steps against the fp32
+    system; and the same fp32 solve issued one system per launch (the
+    reference's CUDA model).  Errors are against the host FP64 Thomas solve,
+    per system, in the CPU leg."""
+    import numpy as np
+    import torch
+
+    import tpr_amd
+    from tpr_amd import batched as B
+
+    nsys, n, s = 256, 65536, 256
+    torch.cuda.set_device(0)
+    t = B.alloc_batch(nsys, n)
+    B.generate("weak", 30303, 0, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
+    h16 = {k: t[k].half() for k in "abc"}
+    torch.cuda.synchronize()
+    ch = args.chunk
+    ws = torch.empty(B.workspace_bytes(B.make_desc(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, chunk=ch)),
+                     dtype=torch.uint8, device="cuda")
+
+    def solve(coef16, refine_steps, method="ptpr"):
+        src = h16 if coef16 else t
+        B.solve_ex(src["a"], src["b"], src["c"], t["d"], t["x"], n, s, method=method, workspace=ws, chunk=ch)
+        for _ in range(refine_steps):
+            B.solve_ex(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method=method, refine=True, workspace=ws,
+                       chunk=ch)
+
+    def per_system():
+        for k in range(nsys):
+            sl = slice(k * n, (k + 1) * n)
+            B.solve_ex(t["a"][sl], t["b"][sl], t["c"][sl], t["d"][sl], t["x"][sl], n, s, workspace=ws)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(reps):
+                fn()
+        g.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    variants = []
+    for coef16 in (False, True):
+        for steps in (0, 1, 2):
+            ms = timed(lambda: solve(coef16, steps), 10)
+            solve(coef16, steps)
+            torch.cuda.synchronize()
+            x = t["x"].cpu().numpy()
+            # algorithmic bytes: solve 20 B/eq (14 with fp16 a, b, c), each
+            # refinement step 24 B/eq (a, b, c, d, x read; x written)
+            alg = nsys * n * ((14 if coef16 else 20) + 24 * steps)
+            variants.append({"name": "%s%s" % ("fp16" if coef16 else "fp32", "+%d refine" % steps if steps else ""),
+                             "ms": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
+                             "GBps": round(alg / (ms / 1e3) / 1e9, 1), "x": x})
+    ms = timed(per_system, 2)
+    variants.append({"name": "fp32, one system per launch", "ms": round(ms, 4),
+                     "systems_per_s": round(nsys / (ms / 1e3), 1),
+                     "GBps": round(20 * nsys * n / (ms / 1e3) / 1e9, 1), "x": None})
+    host = {k: t[k].cpu().numpy() for k in ("a", "b", "c", "d")}
+    out = {"config": "config 3: %d weakly dominant fp32 systems x N=%d, S=%d (b = |a|+|c|+eps, eps ~ U[1e-4,1e-2])"
+                     % (nsys, n, s), "chunk": ch, "variants": variants}
+    if not args.no_cpu:
+        import oracle as O
+        xf = O.thomas_f64_batch(host["a"], host["b"], host["c"], host["d"], n).reshape(nsys, n)
+        for v in variants:
+            if v["x"] is None:
+                continue
+            x = v["x"].reshape(nsys, n).astype(np.float64)
+            err = np.linalg.norm(x - xf, axis=1) / np.linalg.norm(xf, axis=1)
+            v["rel_l2_vs_fp64_thomas_max"] = float(err.max())
+            v["rel_l2_vs_fp64_thomas_median"] = float(np.median(err))
+        kind = "reference" if O.ref_available() else "port"
+        solvef = O.ref_solve_batch if kind == "reference" else O.solve_batch
+        k = 32
+        t0 = time.perf_counter()
+        xc = solvef("ptpr", host["a"][:k * n], host["b"][:k * n], host["c"][:k * n], host["d"][:k * n], n, s, 1)
+        el = time.perf_counter() - t0
+        err = np.linalg.norm(xc.reshape(k, n) - xf[:k], axis=1) / np.linalg.norm(xf[:k], axis=1)
+        out["cpu_baseline"] = {"value": round(k / el, 2), "unit": "systems/s", "cores": O.max_threads(),
+                               "kind": kind, "sample": "%d of the %d systems, one PTPR per thread" % (k, nsys),
+                               "rel_l2_vs_fp64_thomas_max": float(err.max())}
+    for v in variants:
+        v.pop("x", None)
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (systems are independent and "
               "would be sharded by count across GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == 3:
+        config3(args)
+        return
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps
     value = NSYS / (ms_per_step / 1e3)

@@ -108,9 +108,9 @@ def test_workspace_sizing_without_device():
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == 0          # one workgroup per system
     d.n, d.nsys = 65536, 256
     ws = L.tprh_workspace_bytes(ctypes.byref(d))
-    assert ws > 3 * 4 * 65536                                     # scratch for at least one system
+    assert ws == 4 * 256 * 9 * 256 + 256                          # rows, boundary x and x_ed per slice
     d.chunk = 3
-    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * (3 * 65536 + 7 * 256) + 256
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 256
     d.s = 3
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
     d.s, d.refine, d.coef = 256, 1, _lib.STORE_F16                # refinement needs fp32 coefficients

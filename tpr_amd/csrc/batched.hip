@@ -33,7 +33,13 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
     RawPart<E> raw;
     load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
     float xo[E];
-    if constexpr (REFINE) residual_fp64<E, HAS_B>(x, x0, off, n, raw, xo);
+    if constexpr (REFINE) {
+        // every lane of the system loads x here, before the barriers of the
+        // reduced solve; all stores of x come after them
+        float xl, xr;
+        load_x_with_halo<E>(x, x0, off, n, xo, xl, xr);
+        residual_fp64<E, HAS_B>(xo, xl, xr, raw);
+    }
     float A[E], C[E], R[E], X[E];
     normalise<E, MODE>(raw, HAS_B, A, C, R);
     if constexpr (METHOD == 0) {

@@ -68,10 +68,10 @@ tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape
     return TPRH_SUCCESS;
 }
 
-// Systems per round of the multi-workgroup path: inputs (16 B/eq) and
-// scratch (12 B/eq) of a round within 128 MiB, half the Infinity Cache.
+// Systems per round of the multi-workgroup path: the whole batch, up to
+// 64 Mi equations (a 1.75 MB workspace per million slices).
 int default_chunk(int n, int nsys) {
-    long long c = (128LL << 20) / (28LL * n);
+    long long c = (64LL << 20) / n;
     if (c < 1) c = 1;
     if (c > nsys) c = nsys;
     return (int)c;
@@ -102,7 +102,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     pl->m = d.n / d.s;
     pl->chunk = d.chunk > 0 ? (d.chunk < d.nsys ? d.chunk : d.nsys) : default_chunk(d.n, d.nsys);
     if (pl->chunk < 1) pl->chunk = 1;
-    pl->ws = pl->split ? tprh::split_workspace_bytes(d.n, pl->m, pl->chunk) : 0;
+    pl->ws = pl->split ? tprh::split_workspace_bytes(pl->m, pl->chunk) : 0;
     return TPRH_SUCCESS;
 }
 
@@ -134,7 +134,7 @@ tprh_status solve_desc(const tprh_solve_desc& d, hipStream_t stream) {
         p.chunk = pl.chunk;
         p.f16 = d.coef == TPRH_STORE_F16;
         p.refine = d.refine != 0;
-        tprh::split_carve(d.workspace, d.n, pl.m, pl.chunk, &p);
+        tprh::split_carve(d.workspace, pl.m, pl.chunk, &p);
         e = tprh::launch_split(method, mode, p, pl.E, pl.L, stream);
     }
     if (e != hipSuccess) return fail_hip(e, "solver kernel launch");

@@ -32,28 +32,24 @@ struct SplitArgs {
     int n, nsys, chunk;
     bool f16, refine;
     // workspace, for one round of `chunk` systems
-    float *sA, *sC, *sR;  // chunk * n each
-    float* rows;          // chunk * m * 6
-    float* xed;           // chunk * m
+    float* rows;  // chunk * m * kSplitRowWords: boundary rows (and x) per slice
+    float* xed;   // chunk * m: x at the end of every slice
 };
 
-// Bytes of workspace for rounds of `chunk` systems of n equations, m slices.
-inline size_t split_workspace_bytes(int n, int m, int chunk) {
-    return sizeof(float) * (size_t)chunk * (3 * (size_t)n + 7 * (size_t)m) + 256;
+// Floats per slice in the boundary buffer (see split.hip).
+constexpr int kSplitRowWords = 8;
+
+// Bytes of workspace for rounds of `chunk` systems of m slices.
+inline size_t split_workspace_bytes(int m, int chunk) {
+    return sizeof(float) * (size_t)chunk * (kSplitRowWords + 1) * (size_t)m + 256;
 }
 
 // Carves the workspace (256-byte aligned) into the SplitArgs pointers.
-inline void split_carve(void* ws, int n, int m, int chunk, SplitArgs* p) {
+inline void split_carve(void* ws, int m, int chunk, SplitArgs* p) {
     size_t addr = ((size_t)ws + 255) & ~(size_t)255;
     float* f = reinterpret_cast<float*>(addr);
-    p->sA = f;
-    f += (size_t)chunk * n;
-    p->sC = f;
-    f += (size_t)chunk * n;
-    p->sR = f;
-    f += (size_t)chunk * n;
     p->rows = f;
-    f += (size_t)chunk * m * 6;
+    f += (size_t)chunk * m * kSplitRowWords;
     p->xed = f;
 }
 

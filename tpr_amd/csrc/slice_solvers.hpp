@@ -118,18 +118,14 @@ __device__ __forceinline__ void load_raw_t(const CT* __restrict__ a, const CT* _
 }
 
 // Iterative refinement: replaces p.d by the residual r = d - A x of the
-// system at the current x, accumulated in FP64 (products of two floats are
-// exact in double, so the only roundings are the three additions and the
-// final conversion).  x0 is the system's first equation (global index), i0
-// the lane's first equation, n the system size; the lane reads x[i0-1 ..
-// i0+E] (neighbours outside the system count as 0, as a[0] and c[n-1] are
-// outside the matrix).  HAS_B == false: unit diagonal.
+// system at the current x, accumulated in FP64 in the order
+// ((d - b x) - a x_left) - c x_right (products of two floats are exact in
+// double, so the only roundings are the three subtractions and the final
+// conversion).  xo holds the lane's own E values of x; xl / xr are x just
+// before / after them (0 outside the system: a[0] and c[n-1] are outside the
+// matrix).  HAS_B == false: unit diagonal.
 template <int E, bool HAS_B>
-__device__ __forceinline__ void residual_fp64(const float* __restrict__ x, size_t x0, size_t i0, int n,
-                                              RawPart<E>& p, float (&xo)[E]) {
-    load_e<E>(x + i0, xo);
-    const float xl = (i0 > x0) ? x[i0 - 1] : 0.0f;
-    const float xr = (i0 + E < x0 + (size_t)n) ? x[i0 + E] : 0.0f;
+__device__ __forceinline__ void residual_fp64(const float (&xo)[E], float xl, float xr, RawPart<E>& p) {
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const double xm = (k == 0) ? (double)xl : (double)xo[k - 1];
@@ -140,6 +136,16 @@ __device__ __forceinline__ void residual_fp64(const float* __restrict__ x, size_
         acc -= (double)p.c[k] * xp;
         p.d[k] = (float)acc;
     }
+}
+
+// Loads the lane's x values and their outer neighbours from memory (x0: the
+// system's first equation, i0: the lane's first equation).
+template <int E>
+__device__ __forceinline__ void load_x_with_halo(const float* __restrict__ x, size_t x0, size_t i0, int n,
+                                                 float (&xo)[E], float& xl, float& xr) {
+    load_e<E>(x + i0, xo);
+    xl = (i0 > x0) ? x[i0 - 1] : 0.0f;
+    xr = (i0 + E < x0 + (size_t)n) ? x[i0 + E] : 0.0f;
 }
 
 // Divides by the diagonal as the 4-argument setter does (src/ptpr.hpp:80-90);

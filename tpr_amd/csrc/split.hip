@@ -2,22 +2,24 @@
 //
 // The reference needs every block of a system co-resident for its grid-wide
 // barriers and limits n to s^2 (src/ptpr.cu:543,581,655).  Here a system of m
-// slices is solved in three kernels, each an ordinary launch:
-//   K1 stage1_kernel : every lane group runs stage 1 on its slices
-//                      (src/ptpr.cpp:130-190 / src/tpr.cpp:123-216) and
-//                      writes the per-equation stage-3 coefficients to a
-//                      scratch array plus two boundary rows per slice (E_st
-//                      before its update, E_ed);
+// slices is solved by three ordinary launches:
+//   K1 rows_kernel   : every lane group reduces its slices and writes two
+//                      boundary rows per slice (E_st before its update, E_ed);
 //   K2 reduced_kernel: one workgroup per system applies E_st += s1 E_ed
 //                      (src/ptpr.cpp:193-211), builds the m reduced rows
 //                      (:218-241) and solves them in LDS (PCR for PTPR,
 //                      src/pcr.cpp; CR for TPR, src/cr.cpp);
-//   K3 stage3_kernel : every lane group reads back its coefficients and the
-//                      slice-end unknowns and runs stage 3
-//                      (src/ptpr.cpp:263-287 / src/tpr.cpp:277-308).
-// The batch is processed in rounds of `chunk` systems that reuse one scratch
-// area, sized so that a round's inputs and scratch stay in the 256 MB
-// Infinity Cache between K1 and K3.
+//   K3 stage3_kernel : every lane group reloads its slices, redoes stage 1
+//                      and runs stage 3 (src/ptpr.cpp:263-287 /
+//                      src/tpr.cpp:277-308).
+// K1 needs only the first and last row of every slice.  In PTPR's in-slice
+// PCR the final value of a slice's first (last) row depends only on the rows
+// at offsets 0 mod 2^p (-1 mod 2^p) of the previous levels, updated with the
+// same formulas in the same order as TPR's tree stage 1 (src/tpr.cpp:142-204
+// vs src/ptpr.cpp:140-182), so K1 runs the O(S) tree for both methods and its
+// rows are bit-identical to the full PCR's.  Nothing but those rows is
+// stored between the kernels: HBM traffic is 16 B/eq (K1) + 16 B/eq + 4 B/eq
+// (K3), with the stage-1 work of K1 a quarter of a full PTPR stage 1.
 #include <hip/hip_runtime.h>
 
 #include "launch_split.hpp"
@@ -25,66 +27,39 @@
 
 namespace tprh {
 
-// Row layout of the boundary buffer: per slice 6 floats,
-// [E_st.a, E_st.c, E_st.r, E_ed.a, E_ed.c, E_ed.r].
-constexpr int kRowWords = 6;
+// Row layout of the boundary buffer: per slice 8 floats,
+// [E_st.a, E_st.c, E_st.r, E_ed.a, E_ed.c, E_ed.r, x_st, x_ed]; the last two
+// (refinement only) keep the slice's first and last x of the current iterate,
+// so that K3 can rebuild the residual at slice edges after neighbouring
+// workgroups have overwritten x.
+constexpr int kRowWords = kSplitRowWords;
 
-template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
-__global__ __launch_bounds__(256) void stage1_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
-                                                     const CT* __restrict__ c, const float* __restrict__ d,
-                                                     const float* __restrict__ xin, float* __restrict__ sA,
-                                                     float* __restrict__ sC, float* __restrict__ sR,
-                                                     float* __restrict__ rows, int n, int lg_m, long long nslices) {
+template <int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+__global__ __launch_bounds__(256) void rows_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
+                                                   const CT* __restrict__ c, const float* __restrict__ d,
+                                                   const float* __restrict__ xin, float* __restrict__ rows, int n,
+                                                   int lg_m, long long nslices) {
     constexpr int S = E * L;
     const int l = threadIdx.x % L;
     const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;  // slice over the batch
     const bool valid = g < nslices;
     const long long gs = valid ? g : 0;
-    const long long sys = gs >> lg_m;
-    const size_t x0 = (size_t)sys * (size_t)n;
+    const size_t x0 = (size_t)(gs >> lg_m) * (size_t)n;
     const size_t off = (size_t)gs * S + (size_t)l * E;
     RawPart<E> raw;
     load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    float* row = rows + (size_t)gs * kRowWords;
     if constexpr (REFINE) {
-        float xo[E];
-        residual_fp64<E, HAS_B>(xin, x0, off, n, raw, xo);
+        float xo[E], xl, xr;
+        load_x_with_halo<E>(xin, x0, off, n, xo, xl, xr);
+        residual_fp64<E, HAS_B>(xo, xl, xr, raw);
+        if (valid && l == 0) row[6] = xo[0];
+        if (valid && l == L - 1) row[7] = xo[E - 1];
     }
     float A[E], C[E], R[E];
     normalise<E, MODE>(raw, HAS_B, A, C, R);
-    if constexpr (METHOD == 0) {
-        ptpr_stage1<E, L, MODE>(A, C, R, l);
-        if (valid) {
-            store_e<E>(sA + off, A);
-            store_e<E>(sC + off, C);
-            store_e<E>(sR + off, R);
-        }
-    } else {
-        float Ao[E], Co[E], Ro[E];
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            Ao[i] = A[i];
-            Co[i] = C[i];
-            Ro[i] = R[i];
-        }
-        tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
-        // stage-3 coefficients: originals at even offsets, stage-1 values at
-        // odd offsets (src/tpr.cpp:128-132,209-213,381-387)
-        float QA[E], QC[E], QR[E];
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            const bool even = (E % 2 == 0) ? ((i & 1) == 0) : ((l & 1) == 0);
-            QA[i] = even ? Ao[i] : A[i];
-            QC[i] = even ? Co[i] : C[i];
-            QR[i] = even ? Ro[i] : R[i];
-        }
-        if (valid) {
-            store_e<E>(sA + off, QA);
-            store_e<E>(sC + off, QC);
-            store_e<E>(sR + off, QR);
-        }
-    }
+    tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
     if (valid) {
-        float* row = rows + (size_t)gs * kRowWords;
         if (l == 0) {
             row[0] = A[0];
             row[1] = C[0];
@@ -151,42 +126,76 @@ __global__ __launch_bounds__(1024) void reduced_kernel(const float* __restrict__
     for (int j = threadIdx.x; j < m; j += nthr) xed[(size_t)blockIdx.x * m + j] = res[j];
 }
 
-template <int METHOD, int E, int L, int MODE, bool REFINE>
-__global__ __launch_bounds__(256) void stage3_kernel(const float* __restrict__ sA, const float* __restrict__ sC,
-                                                     const float* __restrict__ sR, const float* __restrict__ rows,
-                                                     const float* __restrict__ xed, float* __restrict__ x, int lg_m,
-                                                     long long nslices) {
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+__global__ __launch_bounds__(256) void stage3_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
+                                                     const CT* __restrict__ c, const float* __restrict__ d,
+                                                     const float* __restrict__ rows, const float* __restrict__ xed,
+                                                     float* __restrict__ x, int n, int lg_m, long long nslices) {
     constexpr int S = E * L;
     const int l = threadIdx.x % L;
     const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;
     const bool valid = g < nslices;
     const long long gs = valid ? g : 0;
     const int j = (int)(gs & ((1LL << lg_m) - 1));  // slice inside its system
+    const size_t x0 = (size_t)(gs >> lg_m) * (size_t)n;
     const size_t off = (size_t)gs * S + (size_t)l * E;
+    RawPart<E> raw;
+    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    float xo[E];
+    if constexpr (REFINE) {
+        // neighbours inside the slice belong to this wave, whose loads all
+        // precede its stores; across slice edges, the values K1 saved
+        load_e<E>(x + off, xo);
+        const int m = 1 << lg_m;
+        float xl, xr;
+        if (l > 0) xl = x[off - 1];
+        else xl = (j > 0) ? rows[(size_t)(gs - 1) * kRowWords + 7] : 0.0f;
+        if (l < L - 1) xr = x[off + E];
+        else xr = (j < m - 1) ? rows[(size_t)(gs + 1) * kRowWords + 6] : 0.0f;
+        residual_fp64<E, HAS_B>(xo, xl, xr, raw);
+    }
     float A[E], C[E], R[E], X[E];
-    load_e<E>(sA + off, A);
-    load_e<E>(sC + off, C);
-    load_e<E>(sR + off, R);
+    normalise<E, MODE>(raw, HAS_B, A, C, R);
     const float xe = xed[gs];
     const float xprev = (j > 0) ? xed[gs - 1] : 0.0f;
     if constexpr (METHOD == 0) {
+        ptpr_stage1<E, L, MODE>(A, C, R, l);
         const float* row = rows + (size_t)gs * kRowWords;
         ptpr_stage3<E, L, MODE>(A, C, R, xprev, xe, SliceEnd{row[3], row[4], row[5]}, l, X);
     } else {
-        tpr_stage3<E, L, MODE>(A, C, R, A, C, R, xprev, xe, l, X);
-    }
-    if (valid) {
-        if constexpr (REFINE) {
-            float xo[E];
-            load_e<E>(x + off, xo);
+        float Ao[E], Co[E], Ro[E];
 #pragma unroll
-            for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
+        for (int i = 0; i < E; i++) {
+            Ao[i] = A[i];
+            Co[i] = C[i];
+            Ro[i] = R[i];
         }
-        store_e<E>(x + off, X);
+        tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+        tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xe, l, X);
     }
+    if constexpr (REFINE) {
+#pragma unroll
+        for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
+    }
+    if (valid) store_e<E>(x + off, X);
 }
 
 namespace {
+
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+void launch_round(const SplitArgs& p, size_t base, int grid, int threads, int lg_m, long long nslices, int cs, int m,
+                  int rthreads, size_t rshmem, hipStream_t stream) {
+    const CT* a = static_cast<const CT*>(p.a) + base;
+    const CT* c = static_cast<const CT*>(p.c) + base;
+    const CT* b = HAS_B ? static_cast<const CT*>(p.b) + base : nullptr;
+    const float* d = p.d + base;
+    float* x = p.x + base;
+    hipLaunchKernelGGL((rows_kernel<E, L, MODE, HAS_B, CT, REFINE>), dim3(grid), dim3(threads), 0, stream, a, b, c, d,
+                       x, p.rows, p.n, lg_m, nslices);
+    hipLaunchKernelGGL((reduced_kernel<METHOD>), dim3(cs), dim3(rthreads), rshmem, stream, p.rows, p.xed, m);
+    hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE>), dim3(grid), dim3(threads), 0, stream,
+                       a, b, c, d, p.rows, p.xed, x, p.n, lg_m, nslices);
+}
 
 template <int METHOD, int E, int L, int MODE>
 hipError_t launch_split_el(const SplitArgs& p, hipStream_t stream) {
@@ -210,51 +219,16 @@ hipError_t launch_split_el(const SplitArgs& p, hipStream_t stream) {
         const size_t base = (size_t)s0 * p.n;
         const long long nslices = (long long)cs * m;
         const int grid = (int)((nslices + slices_per_block - 1) / slices_per_block);
-        float* x = p.x + base;
-        const float* d = p.d + base;
+        const bool hb = p.b != nullptr;
         if (p.f16) {
-            const _Float16* a = static_cast<const _Float16*>(p.a) + base;
-            const _Float16* c = static_cast<const _Float16*>(p.c) + base;
-            const _Float16* b = p.b ? static_cast<const _Float16*>(p.b) + base : nullptr;
-            if (b) {
-                hipLaunchKernelGGL((stage1_kernel<METHOD, E, L, MODE, true, _Float16, false>), dim3(grid),
-                                   dim3(threads), 0, stream, a, b, c, d, nullptr, p.sA, p.sC, p.sR, p.rows, p.n, lg_m,
-                                   nslices);
-            } else {
-                hipLaunchKernelGGL((stage1_kernel<METHOD, E, L, MODE, false, _Float16, false>), dim3(grid),
-                                   dim3(threads), 0, stream, a, b, c, d, nullptr, p.sA, p.sC, p.sR, p.rows, p.n,
-                                   lg_m, nslices);
-            }
+            if (hb) launch_round<METHOD, E, L, MODE, true, _Float16, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
+            else launch_round<METHOD, E, L, MODE, false, _Float16, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
+        } else if (p.refine) {
+            if (hb) launch_round<METHOD, E, L, MODE, true, float, true>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
+            else launch_round<METHOD, E, L, MODE, false, float, true>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
         } else {
-            const float* a = static_cast<const float*>(p.a) + base;
-            const float* c = static_cast<const float*>(p.c) + base;
-            const float* b = p.b ? static_cast<const float*>(p.b) + base : nullptr;
-            if (p.refine) {
-                if (b) {
-                    hipLaunchKernelGGL((stage1_kernel<METHOD, E, L, MODE, true, float, true>), dim3(grid),
-                                       dim3(threads), 0, stream, a, b, c, d, x, p.sA, p.sC, p.sR, p.rows, p.n, lg_m,
-                                       nslices);
-                } else {
-                    hipLaunchKernelGGL((stage1_kernel<METHOD, E, L, MODE, false, float, true>), dim3(grid),
-                                       dim3(threads), 0, stream, a, b, c, d, x, p.sA, p.sC, p.sR, p.rows, p.n, lg_m,
-                                       nslices);
-                }
-            } else if (b) {
-                hipLaunchKernelGGL((stage1_kernel<METHOD, E, L, MODE, true, float, false>), dim3(grid), dim3(threads),
-                                   0, stream, a, b, c, d, nullptr, p.sA, p.sC, p.sR, p.rows, p.n, lg_m, nslices);
-            } else {
-                hipLaunchKernelGGL((stage1_kernel<METHOD, E, L, MODE, false, float, false>), dim3(grid),
-                                   dim3(threads), 0, stream, a, b, c, d, nullptr, p.sA, p.sC, p.sR, p.rows, p.n, lg_m,
-                                   nslices);
-            }
-        }
-        hipLaunchKernelGGL((reduced_kernel<METHOD>), dim3(cs), dim3(rthreads), rshmem, stream, p.rows, p.xed, m);
-        if (p.refine) {
-            hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, true>), dim3(grid), dim3(threads), 0, stream, p.sA,
-                               p.sC, p.sR, p.rows, p.xed, x, lg_m, nslices);
-        } else {
-            hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, false>), dim3(grid), dim3(threads), 0, stream, p.sA,
-                               p.sC, p.sR, p.rows, p.xed, x, lg_m, nslices);
+            if (hb) launch_round<METHOD, E, L, MODE, true, float, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
+            else launch_round<METHOD, E, L, MODE, false, float, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
