@@ -52,8 +52,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="config 3: systems per round of the split path")
-    ap.add_argument("--config", type=int, default=1, choices=[1, 3],
-                    help="1: the headline batch (default); 3: 256 x N=65536 weakly dominant, precision variants")
+    ap.add_argument("--config", type=int, default=1, choices=[1, 2, 3],
+                    help="1: the headline batch (default); 2: 20000 systems of uneven N in one persistent launch; "
+                         "3: 256 x N=65536 weakly dominant, precision variants")
+    ap.add_argument("--retune", action="store_true",
+                    help="config 2: run the autotuner now instead of loading the committed schedule")
     return ap.parse_args()
 
 
@@ -252,6 +255,139 @@ def config3(args):
     print(json.dumps(out))
 
 
+def config2(args):
+    """20000 systems, N = 2^U{8..14} (log-uniform over the powers of two),
+    diagonally dominant generator, one persistent launch per step, replayed
+    from a captured graph.  Schedules compared on the same batch:
+    * autotuned: s and slice shape per size class from tpr_amd.autotune;
+    * config-1 schedule: s = min(N, 256) for every class, same kernel;
+    * grouped: the systems sorted by size and one batched launch per size
+      class at s = min(N, 256) (the plain batched kernel, 7 launches).
+    Algorithmic bytes: 20 B per equation, ~92.9 M equations per step."""
+    import numpy as np
+    import torch
+
+    from tpr_amd import autotune
+    from tpr_amd import batched as B
+    from tpr_amd.uneven import UnevenPlan, log_uniform_sizes
+
+    nsys = 20000
+    torch.cuda.set_device(0)
+    sizes = log_uniform_sizes(nsys, 8, 14, seed=2020)
+    if args.retune or not os.path.exists(autotune.DEFAULT_PATH):
+        table = autotune.tune(method=args.method)
+        tuned_from = "tuned in this run"
+    else:
+        table = autotune.load()
+        tuned_from = os.path.relpath(autotune.DEFAULT_PATH, ROOT)
+    sched = autotune.schedule_of(table)
+    plans = {"autotuned": UnevenPlan(sizes, args.method, args.arith, sched),
+             "config-1 schedule": UnevenPlan(sizes, args.method, args.arith, None)}
+    tot = plans["autotuned"].equations
+    a, b, c, d, x, xt = (torch.empty(tot, device="cuda") for _ in range(6))
+    plans["autotuned"].generate("dominant", 4242, 0, a, b, c, d, xt)
+    # class-sorted copy for the grouped launches (stable order inside a class)
+    order = np.argsort(sizes, kind="stable")
+    srt = UnevenPlan(sizes[order], args.method, args.arith, None)
+    idx = np.concatenate([np.arange(plans["autotuned"].offset(k), plans["autotuned"].offset(k + 1))
+                          for k in order])
+    idx_t = torch.from_numpy(idx).cuda()
+    sa, sb, sc, sd = (v[idx_t].contiguous() for v in (a, b, c, d))
+    sx = torch.empty_like(sa)
+    groups = []
+    for n in sorted(set(sizes.tolist())):
+        ks = np.nonzero(sizes[order] == n)[0]
+        lo, hi = srt.offset(int(ks[0])), srt.offset(int(ks[-1]) + 1)
+        groups.append((n, lo, hi))
+    torch.cuda.synchronize()
+
+    def grouped():
+        for n, lo, hi in groups:
+            B.solve_batched(sa[lo:hi], sb[lo:hi], sc[lo:hi], sd[lo:hi], sx[lo:hi], n, min(n, 256),
+                            method=args.method, arith=args.arith)
+
+    runs = {name: (lambda p=p: p.solve(a, b, c, d, x)) for name, p in plans.items()}
+    runs["grouped launches"] = grouped
+    steps = args.steps
+    res = {}
+    for name, fn in runs.items():
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(steps):
+                fn()
+        g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        gbps = 20.0 * tot / (ms / 1e3) / 1e9
+        res[name] = {"ms_per_step": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
+                     "GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
+    # accuracy: the autotuned answers against the oracle (same s per class)
+    # and against the truth, on a sample of systems
+    plans["autotuned"].solve(a, b, c, d, x)
+    torch.cuda.synchronize()
+    h = {k: v.cpu().numpy() for k, v in (("a", a), ("b", b), ("c", c), ("d", d), ("x", x), ("xt", xt))}
+    best = res["autotuned"]
+    out = {
+        "metric": METRIC, "value": best["systems_per_s"], "unit": "systems/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "uneven batch: %d fp32 systems, N = 2^U{8..14} (%d equations), diagonally dominant "
+                               "generator, one persistent launch per step replayed from a HIP graph" % (nsys, tot),
+                   "method": args.method, "arith": args.arith, "schedule": {str(k): v for k, v in sched.items()},
+                   "schedule_source": tuned_from,
+                   "class_counts": {str(n): int((sizes == n).sum()) for n in sorted(set(sizes.tolist()))}},
+        "roofline": {"bound": "hbm", "achieved": best["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": best["hbm_frac"], "traffic": None},
+        "schedules": res,
+    }
+    if not args.no_cpu:
+        import oracle as O
+        pa = plans["autotuned"]
+        rng = np.random.default_rng(7)
+        pick = rng.choice(nsys, 400, replace=False)
+        worst_o = worst_t = 0.0
+        for k in pick:
+            lo, hi = pa.offset(int(k)), pa.offset(int(k) + 1)
+            n = int(sizes[k])
+            ref = O.solve_batch(args.method, h["a"][lo:hi], h["b"][lo:hi], h["c"][lo:hi], h["d"][lo:hi], n,
+                                sched.get(n, (min(n, 256),))[0])
+            worst_o = max(worst_o, O.rel_l2(h["x"][lo:hi], ref))
+            worst_t = max(worst_t, O.rel_l2(h["x"][lo:hi], h["xt"][lo:hi]))
+        out["accuracy"] = {"systems_checked": len(pick), "max_rel_l2_vs_oracle_same_s": worst_o,
+                           "max_rel_l2_vs_truth": worst_t, "tolerance": 1e-5}
+        kind = "reference" if O.ref_available() else "port"
+        solvef = O.ref_solve_batch if kind == "reference" else O.solve_batch
+        sample = []
+        for n, lo, hi in groups:
+            cnt = min(16, (hi - lo) // n)
+            hs = slice(lo, lo + cnt * n)
+            sample.append((n, cnt, [v[hs].cpu().numpy() for v in (sa, sb, sc, sd)]))
+        t0 = time.perf_counter()
+        done = eqs = 0
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            for n, cnt, (ha, hb, hc, hd) in sample:
+                solvef(args.method, ha, hb, hc, hd, n, min(n, 256), 1)
+                done += cnt
+                eqs += cnt * n
+        el = time.perf_counter() - t0
+        # the sample has 16 systems of every class; weight to the batch's mix
+        mean_n = tot / nsys
+        out["cpu_baseline"] = {"value": round(eqs / el / mean_n, 1), "unit": "systems/s", "cores": O.max_threads(),
+                               "kind": kind,
+                               "sample": "%d systems (16 per size class per pass) in %.1f s, one %s per thread; "
+                                         "systems/s = equations/s over the batch's mean N" %
+                                         (done, el, args.method.upper())}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -260,6 +396,9 @@ def main():
         sys.exit(2)
     if args.config == 3:
         config3(args)
+        return
+    if args.config == 2:
+        config2(args)
         return
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps

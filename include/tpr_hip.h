@@ -166,6 +166,29 @@ typedef struct tprh_solve_desc {
 size_t tprh_workspace_bytes(const tprh_solve_desc *desc);
 tprh_status tprh_solve_ex(const tprh_solve_desc *desc, void *stream);
 
+/* ---- batches of systems of different sizes ---------------------------------
+ * One persistent launch for the whole batch.  n[k] (host array) is the size
+ * of system k; the systems are stored back to back in this order in a, b, c,
+ * d and x.  schedule: NULL, or 32 triples (s, E, L) indexed by log2(n) giving
+ * the slice size and slice shape of every size class (0, 0, 0 = default:
+ * s = min(n, 256) with its default shape); a system's result depends on its
+ * class's entry only, never on the rest of the batch.  Sizes: powers of two
+ * from 16 with n / s * L <= 1024.  Creating the plan allocates its device
+ * tables; tprh_uneven_solve allocates nothing and is graph-capturable (two
+ * kernels: a counter reset and the persistent solver). */
+typedef struct tprh_uneven_plan tprh_uneven_plan;
+tprh_status tprh_uneven_plan_create(tprh_uneven_plan **out, tprh_method method, tprh_arith arith, const int *n,
+                                    int nsys, const int *schedule);
+tprh_status tprh_uneven_solve(const tprh_uneven_plan *plan, const float *a, const float *b, const float *c,
+                              const float *d, float *x, void *stream);
+/* Element offset of system k (host value), and the total number of equations. */
+long long tprh_uneven_offset(const tprh_uneven_plan *plan, int k);
+long long tprh_uneven_equations(const tprh_uneven_plan *plan);
+/* The generator for the plan's layout: system k gets id sys0 + k. */
+tprh_status tprh_uneven_generate(const tprh_uneven_plan *plan, tprh_gen kind, uint64_t seed, uint64_t sys0,
+                                 float *a, float *b, float *c, float *d, float *xt, void *stream);
+tprh_status tprh_uneven_plan_destroy(tprh_uneven_plan *plan);
+
 /* Synthetic systems with ids sys0 .. sys0+nsys-1 written back to back into
  * device arrays a, b, c, d (and xt unless NULL); bit-identical to the
  * oracle's generator on the host.  Asynchronous on `stream`. */

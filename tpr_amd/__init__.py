@@ -8,4 +8,5 @@ classes and CUDA entry points (``PTPR``, ``TPR``, ``PCR``, ``CR``, ``ptpr_cu``,
 """
 from ._lib import TprError, lib  # noqa: F401
 from .batched import alloc_batch, generate, shape_supported, solve_batched, solve_ex, workspace_bytes  # noqa: F401
+from .uneven import UnevenPlan  # noqa: F401
 from .solvers import CR, PCR, PTPR, TPR, cr_cu, pcr_cu, ptpr_cu, tpr_cu  # noqa: F401

@@ -65,6 +65,12 @@ class SolveDesc(ctypes.Structure):
 SIGNATURES.update({
     "tprh_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(SolveDesc)]),
     "tprh_solve_ex": (_i, [ctypes.POINTER(SolveDesc), _vp]),
+    "tprh_uneven_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _vp, _i, _vp]),
+    "tprh_uneven_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tprh_uneven_offset": (ctypes.c_longlong, [_vp, _i]),
+    "tprh_uneven_equations": (ctypes.c_longlong, [_vp]),
+    "tprh_uneven_generate": (_i, [_vp, _i, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tprh_uneven_plan_destroy": (_i, [_vp]),
 })
 
 STORE_F32, STORE_F16 = 0, 1

@@ -1,10 +1,13 @@
-// batched.hip -- batched PTPR / TPR: many independent systems of the same
-// size, one group of lanes per system, one launch for the whole batch.
+// batched_impl.hpp -- batched PTPR / TPR: many independent systems of the
+// same size, one group of lanes per system, one launch for the whole batch.
+// Included by batched_ptpr.hip and batched_tpr.hip, one method each, so the
+// two halves compile in parallel.
 //
 // This replaces the reference's one-system-per-launch cooperative kernels
 // (src/ptpr.cu:48-150, src/tpr.cu:44-156): a system lives inside one
 // workgroup, so there is no grid-wide barrier and no co-residency limit, and
 // the reduced system is solved by the system's own workgroup.
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
@@ -115,11 +118,9 @@ hipError_t launch_mode(const BatchArgs& p, int E, int L, hipStream_t stream) {
 
 }  // namespace
 
-hipError_t launch_batched(int method, int mode, const BatchArgs& p, int E, int L, hipStream_t stream) {
-    if (method == 0) {
-        return mode == kExact ? launch_mode<0, kExact>(p, E, L, stream) : launch_mode<0, kFast>(p, E, L, stream);
-    }
-    return mode == kExact ? launch_mode<1, kExact>(p, E, L, stream) : launch_mode<1, kFast>(p, E, L, stream);
+template <int METHOD>
+hipError_t launch_batched_method(int mode, const BatchArgs& p, int E, int L, hipStream_t stream) {
+    return mode == kExact ? launch_mode<METHOD, kExact>(p, E, L, stream) : launch_mode<METHOD, kFast>(p, E, L, stream);
 }
 
 }  // namespace tprh

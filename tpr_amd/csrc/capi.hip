@@ -3,8 +3,10 @@
 // device buffers of the host-pointer entry points, error reporting.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
@@ -14,6 +16,7 @@
 #include "launch.hpp"
 #include "launch_gen.hpp"
 #include "launch_split.hpp"
+#include "launch_uneven.hpp"
 
 namespace {
 
@@ -264,6 +267,24 @@ int ilog2i(int v) {
 
 }  // namespace
 
+struct tprh_uneven_plan {
+    tprh_method method;
+    tprh_arith arith;
+    int nsys = 0;
+    int ntasks = 0;
+    int grid = 0;
+    size_t lds_bytes = 0;
+    long long total = 0;
+    std::vector<long long> h_offsets;
+    // one device allocation: tasks | sys_list | offsets | sizes | counter
+    void* dev = nullptr;
+    tprh::UnevenTask* tasks = nullptr;
+    int* sys_list = nullptr;
+    long long* offsets = nullptr;
+    int* sizes = nullptr;
+    int* counter = nullptr;
+};
+
 struct tprh_solver {
     tprh_method method;
     tprh_arith arith;
@@ -422,6 +443,153 @@ size_t tprh_workspace_bytes(const tprh_solve_desc* desc) {
 tprh_status tprh_solve_ex(const tprh_solve_desc* desc, void* stream) {
     if (!desc) return fail(TPRH_ERROR_INVALID_VALUE, "null descriptor");
     return solve_desc(*desc, (hipStream_t)stream);
+}
+
+tprh_status tprh_uneven_plan_create(tprh_uneven_plan** out, tprh_method method, tprh_arith arith, const int* n,
+                                    int nsys, const int* schedule) {
+    if (!out) return fail(TPRH_ERROR_INVALID_VALUE, "null plan pointer");
+    *out = nullptr;
+    if (method != TPRH_PTPR && method != TPRH_TPR)
+        return fail(TPRH_ERROR_INVALID_VALUE, "method must be TPRH_PTPR or TPRH_TPR");
+    if (arith != TPRH_ARITH_FAST && arith != TPRH_ARITH_EXACT) return fail(TPRH_ERROR_INVALID_VALUE, "bad arith");
+    if (nsys < 0 || (nsys > 0 && !n)) return fail(TPRH_ERROR_INVALID_VALUE, "bad system list");
+    struct Cls {
+        int s, E, L, m;
+    };
+    std::map<int, Cls, std::greater<int>> classes;  // largest systems first
+    std::map<int, std::vector<int>> members;
+    std::vector<long long> offsets((size_t)nsys + 1, 0);
+    for (int k = 0; k < nsys; k++) {
+        const int nk = n[k];
+        if (nk < 16 || !is_pow2(nk)) return fail(TPRH_ERROR_INVALID_VALUE, "system sizes must be powers of two >= 16");
+        offsets[k + 1] = offsets[k] + nk;
+        if (!classes.count(nk)) {
+            const int lg = __builtin_ctz((unsigned)nk);
+            int s = nk < 256 ? nk : 256, E = 0, L = 0;
+            bool expl = false;
+            if (schedule && schedule[3 * lg] != 0) {
+                s = schedule[3 * lg];
+                E = schedule[3 * lg + 1];
+                L = schedule[3 * lg + 2];
+                expl = E != 0 || L != 0;
+            }
+            bool split = false;
+            tprh_status st = check_ns(nk, s, 1, &E, &L, expl, false, &split);
+            if (st != TPRH_SUCCESS) return st;
+            if (tprh::uneven_shape_id(E, L) < 0)
+                return fail(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the uneven-batch kernel");
+            classes[nk] = Cls{s, E, L, nk / s};
+        }
+        members[nk].push_back(k);
+    }
+    std::vector<tprh::UnevenTask> tasks;
+    std::vector<int> sys_list;
+    size_t lds_max = 64;
+    for (const auto& kv : classes) {
+        const Cls& c = kv.second;
+        const int T = c.m * c.L;
+        const int per = 1024 / T;
+        const std::vector<int>& mem = members[kv.first];
+        for (size_t i = 0; i < mem.size(); i += per) {
+            tprh::UnevenTask tk;
+            tk.shape = tprh::uneven_shape_id(c.E, c.L);
+            tk.m = c.m;
+            tk.lg_lanes = __builtin_ctz((unsigned)T);
+            tk.first = (int)sys_list.size();
+            tk.count = (int)std::min<size_t>(per, mem.size() - i);
+            for (int q = 0; q < tk.count; q++) sys_list.push_back(mem[i + q]);
+            tasks.push_back(tk);
+        }
+        lds_max = std::max(lds_max, sizeof(float) * (size_t)per * tprh::lds_words_per_system(c.m));
+    }
+    tprh_uneven_plan* pl = new (std::nothrow) tprh_uneven_plan();
+    if (!pl) return fail(TPRH_ERROR_OUT_OF_MEMORY, "plan allocation");
+    pl->method = method;
+    pl->arith = arith;
+    pl->nsys = nsys;
+    pl->ntasks = (int)tasks.size();
+    pl->lds_bytes = lds_max;
+    pl->total = offsets[nsys];
+    pl->h_offsets = offsets;
+    const size_t bt = sizeof(tprh::UnevenTask) * tasks.size();
+    const size_t bs = sizeof(int) * sys_list.size();
+    const size_t bo = sizeof(long long) * offsets.size();
+    const size_t bn = sizeof(int) * (size_t)nsys;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t total = al(bt) + al(bs) + al(bo) + al(bn) + 256;
+    hipError_t e = hipMalloc(&pl->dev, total);
+    if (e != hipSuccess) {
+        delete pl;
+        return fail_hip(e, "hipMalloc(uneven plan)");
+    }
+    char* base = static_cast<char*>(pl->dev);
+    pl->tasks = reinterpret_cast<tprh::UnevenTask*>(base);
+    pl->sys_list = reinterpret_cast<int*>(base + al(bt));
+    pl->offsets = reinterpret_cast<long long*>(base + al(bt) + al(bs));
+    pl->sizes = reinterpret_cast<int*>(base + al(bt) + al(bs) + al(bo));
+    pl->counter = reinterpret_cast<int*>(base + al(bt) + al(bs) + al(bo) + al(bn));
+    std::vector<int> sizes(n, n + nsys);
+    if ((bt && (e = hipMemcpy(pl->tasks, tasks.data(), bt, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (bs && (e = hipMemcpy(pl->sys_list, sys_list.data(), bs, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = hipMemcpy(pl->offsets, offsets.data(), bo, hipMemcpyHostToDevice)) != hipSuccess ||
+        (bn && (e = hipMemcpy(pl->sizes, sizes.data(), bn, hipMemcpyHostToDevice)) != hipSuccess)) {
+        (void)hipFree(pl->dev);
+        delete pl;
+        return fail_hip(e, "hipMemcpy(uneven plan)");
+    }
+    int dev = 0, cus = 0, occ = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+        (e = tprh::uneven_occupancy(method == TPRH_PTPR ? 0 : 1, arith == TPRH_ARITH_EXACT ? 1 : 0, lds_max, &occ)) !=
+            hipSuccess) {
+        (void)hipFree(pl->dev);
+        delete pl;
+        return fail_hip(e, "device query (uneven plan)");
+    }
+    if (occ < 1) occ = 1;
+    pl->grid = std::max(1, std::min(cus * occ, pl->ntasks));
+    *out = pl;
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_uneven_solve(const tprh_uneven_plan* plan, const float* a, const float* b, const float* c,
+                              const float* d, float* x, void* stream) {
+    if (!plan) return fail(TPRH_ERROR_INVALID_VALUE, "null plan");
+    if (plan->ntasks == 0) return TPRH_SUCCESS;
+    if (!a || !c || !d || !x) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
+    tprh::UnevenLaunch p{plan->tasks, plan->ntasks, plan->counter, plan->sys_list, plan->offsets, a, b, c, d, x,
+                         plan->grid, plan->lds_bytes};
+    hipError_t e = tprh::launch_uneven(plan->method == TPRH_PTPR ? 0 : 1, plan->arith == TPRH_ARITH_EXACT ? 1 : 0, p,
+                                       (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "uneven kernel launch");
+    return TPRH_SUCCESS;
+}
+
+long long tprh_uneven_offset(const tprh_uneven_plan* plan, int k) {
+    if (!plan || k < 0 || k > plan->nsys) return -1;
+    return plan->h_offsets[k];
+}
+
+long long tprh_uneven_equations(const tprh_uneven_plan* plan) { return plan ? plan->total : -1; }
+
+tprh_status tprh_uneven_generate(const tprh_uneven_plan* plan, tprh_gen kind, uint64_t seed, uint64_t sys0, float* a,
+                                 float* b, float* c, float* d, float* xt, void* stream) {
+    if (!plan) return fail(TPRH_ERROR_INVALID_VALUE, "null plan");
+    if (kind < TPRH_GEN_DOMINANT || kind > TPRH_GEN_RANDOM_RHS) return fail(TPRH_ERROR_INVALID_VALUE, "bad kind");
+    if (!a || !b || !c || !d) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
+    if (plan->nsys == 0) return TPRH_SUCCESS;
+    hipError_t e = tprh::launch_generate_uneven((int)kind, seed, sys0, plan->nsys, plan->sizes, plan->offsets, a, b, c,
+                                                d, xt, (hipStream_t)stream);
+    if (e != hipSuccess) return fail_hip(e, "generate kernel launch");
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_uneven_plan_destroy(tprh_uneven_plan* plan) {
+    if (plan) {
+        if (plan->dev) (void)hipFree(plan->dev);
+        delete plan;
+    }
+    return TPRH_SUCCESS;
 }
 
 tprh_status tprh_generate(tprh_gen kind, uint64_t seed, uint64_t sys0, int nsys, int n, float* a, float* b, float* c,

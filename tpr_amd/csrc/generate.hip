@@ -47,6 +47,39 @@ __device__ __forceinline__ void gen_coef(uint64_t key, int i, int n, int kind, f
     b = (fabsf(ai) + fabsf(ci)) + extra;
 }
 
+// Equation i of system `sys` (0 <= i < n) at array position g.
+__device__ __forceinline__ void generate_one(int kind, uint64_t key, int i, int n, size_t g, float* __restrict__ a,
+                                             float* __restrict__ b, float* __restrict__ c, float* __restrict__ d,
+                                             float* __restrict__ xt) {
+    if (kind == 2 || kind == 3) {
+        float ai = (float)(-1.0 / 6.0), ci = (float)(-1.0 / 6.0);
+        if (i == 0) ai = 0.0f;
+        if (i == n - 1) ci = 0.0f;
+        a[g] = ai;
+        c[g] = ci;
+        b[g] = 1.0f;
+        d[g] = kind == 2 ? (float)(1.0 * (i + 1)) : u_pm1(draw24(key, kStRhs, (uint64_t)i));
+        return;
+    }
+    float ai, bi, ci;
+    gen_coef(key, i, n, kind, ai, bi, ci);
+    const float xi = u_pm1(draw24(key, kStXt, (uint64_t)i));
+    double acc = (double)bi * (double)xi;
+    if (i > 0) {
+        const float xl = u_pm1(draw24(key, kStXt, (uint64_t)(i - 1)));
+        acc = (double)ai * (double)xl + acc;
+    }
+    if (i < n - 1) {
+        const float xr = u_pm1(draw24(key, kStXt, (uint64_t)(i + 1)));
+        acc = acc + (double)ci * (double)xr;
+    }
+    a[g] = ai;
+    b[g] = bi;
+    c[g] = ci;
+    d[g] = (float)acc;
+    if (xt) xt[g] = xi;
+}
+
 __global__ void generate_kernel(int kind, uint64_t seed, uint64_t sys0, int nsys, int n, float* __restrict__ a,
                                 float* __restrict__ b, float* __restrict__ c, float* __restrict__ d,
                                 float* __restrict__ xt) {
@@ -55,33 +88,20 @@ __global__ void generate_kernel(int kind, uint64_t seed, uint64_t sys0, int nsys
         const int k = (int)(g / (size_t)n);
         const int i = (int)(g - (size_t)k * n);
         const uint64_t key = mix64(seed + kGolden * (sys0 + (uint64_t)k + 1));
-        if (kind == 2 || kind == 3) {
-            float ai = (float)(-1.0 / 6.0), ci = (float)(-1.0 / 6.0);
-            if (i == 0) ai = 0.0f;
-            if (i == n - 1) ci = 0.0f;
-            a[g] = ai;
-            c[g] = ci;
-            b[g] = 1.0f;
-            d[g] = kind == 2 ? (float)(1.0 * (i + 1)) : u_pm1(draw24(key, kStRhs, (uint64_t)i));
-            continue;
-        }
-        float ai, bi, ci;
-        gen_coef(key, i, n, kind, ai, bi, ci);
-        const float xi = u_pm1(draw24(key, kStXt, (uint64_t)i));
-        double acc = (double)bi * (double)xi;
-        if (i > 0) {
-            const float xl = u_pm1(draw24(key, kStXt, (uint64_t)(i - 1)));
-            acc = (double)ai * (double)xl + acc;
-        }
-        if (i < n - 1) {
-            const float xr = u_pm1(draw24(key, kStXt, (uint64_t)(i + 1)));
-            acc = acc + (double)ci * (double)xr;
-        }
-        a[g] = ai;
-        b[g] = bi;
-        c[g] = ci;
-        d[g] = (float)acc;
-        if (xt) xt[g] = xi;
+        generate_one(kind, key, i, n, g, a, b, c, d, xt);
+    }
+}
+
+// One workgroup per system; system k has n[k] equations at offsets[k].
+__global__ void generate_uneven_kernel(int kind, uint64_t seed, uint64_t sys0, int nsys, const int* __restrict__ n,
+                                       const long long* __restrict__ offsets, float* __restrict__ a,
+                                       float* __restrict__ b, float* __restrict__ c, float* __restrict__ d,
+                                       float* __restrict__ xt) {
+    for (int k = blockIdx.x; k < nsys; k += gridDim.x) {
+        const int nk = n[k];
+        const size_t base = (size_t)offsets[k];
+        const uint64_t key = mix64(seed + kGolden * (sys0 + (uint64_t)k + 1));
+        for (int i = threadIdx.x; i < nk; i += blockDim.x) generate_one(kind, key, i, nk, base + i, a, b, c, d, xt);
     }
 }
 
@@ -104,6 +124,16 @@ hipError_t launch_generate(int kind, uint64_t seed, uint64_t sys0, int nsys, int
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, kind, seed, sys0, nsys, n, a,
+                       b, c, d, xt);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate_uneven(int kind, uint64_t seed, uint64_t sys0, int nsys, const int* n,
+                                  const long long* offsets, float* a, float* b, float* c, float* d, float* xt,
+                                  hipStream_t stream) {
+    const int blocks = nsys < 65536 ? nsys : 65536;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(generate_uneven_kernel, dim3(blocks), dim3(256), 0, stream, kind, seed, sys0, nsys, n, offsets, a,
                        b, c, d, xt);
     return hipGetLastError();
 }

@@ -6,6 +6,9 @@
 
 namespace tprh {
 
+// LDS words per system for the slice-boundary exchange and the reduced solve.
+__host__ __device__ constexpr int lds_words_per_system(int m) { return 13 * m; }
+
 // Slice shapes (E equations per lane, L lanes per slice) compiled into the
 // library, S = E * L.  The default shape for each S is chosen by
 // default_shape(); the others are alternatives for the autotuner.
@@ -71,5 +74,7 @@ struct BatchArgs {
 
 // method 0 = PTPR, 1 = TPR; mode 0 = fast, 1 = exact.
 hipError_t launch_batched(int method, int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
+hipError_t launch_batched_ptpr(int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
+hipError_t launch_batched_tpr(int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
 
 }  // namespace tprh

@@ -21,12 +21,10 @@
 #pragma once
 #include <float.h>
 
+#include "launch.hpp"
 #include "tpr_device.hpp"
 
 namespace tprh {
-
-// LDS words per system for the slice-boundary exchange and the reduced solve.
-__host__ __device__ constexpr int lds_words_per_system(int m) { return 13 * m; }
 
 // ---------------------------------------------------------------------------
 // Loads and stores of E consecutive floats.

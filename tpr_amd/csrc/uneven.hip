@@ -1,0 +1,109 @@
+// uneven.hip -- batches of systems of different sizes in one persistent launch.
+//
+// A plan (built on the host, see capi.hip) sorts the systems into size
+// classes, gives every class a slice size s and slice shape (E, L) from a
+// per-class table (the autotuner's), and cuts each class into work items of
+// as many systems as one 1024-lane workgroup holds.  The persistent kernel
+// keeps one workgroup per resident slot; each workgroup takes the next work
+// item from an atomic counter (items are ordered largest systems first) and
+// solves it with the shape's code path, until the items run out.  Every
+// system's result depends only on its own size class's (s, E, L), never on
+// the batch around it.
+#include <hip/hip_runtime.h>
+
+#include "launch_uneven.hpp"
+#include "slice_solvers.hpp"
+
+namespace tprh {
+
+template <int METHOD, int MODE, bool HAS_B, int E, int L>
+__device__ __forceinline__ void run_task(const UnevenTask& tk, const int* __restrict__ sys_list,
+                                         const long long* __restrict__ offsets, const float* __restrict__ a,
+                                         const float* __restrict__ b, const float* __restrict__ c,
+                                         const float* __restrict__ d, float* __restrict__ x, float* lds) {
+    const int m = tk.m;
+    const int n = m * E * L;
+    const int slot = threadIdx.x >> tk.lg_lanes;
+    const int t = threadIdx.x & ((1 << tk.lg_lanes) - 1);
+    const bool valid = slot < tk.count;
+    const int sys = sys_list[tk.first + (valid ? slot : 0)];
+    const size_t x0 = (size_t)offsets[sys];
+    const size_t off = x0 + (size_t)t * E;
+    RawPart<E> raw;
+    load_raw<E>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    float A[E], C[E], R[E], X[E];
+    normalise<E, MODE>(raw, HAS_B, A, C, R);
+    float* my = lds + slot * lds_words_per_system(m);
+    if constexpr (METHOD == 0) {
+        ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
+    } else {
+        tpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
+    }
+    (void)n;
+    if (valid) store_e<E>(x + off, X);
+}
+
+__global__ void zero_counter_kernel(int* counter) { *counter = 0; }
+
+template <int METHOD, int MODE, bool HAS_B>
+__global__ __launch_bounds__(1024) void uneven_kernel(const UnevenTask* __restrict__ tasks, int ntasks,
+                                                      int* __restrict__ counter, const int* __restrict__ sys_list,
+                                                      const long long* __restrict__ offsets,
+                                                      const float* __restrict__ a, const float* __restrict__ b,
+                                                      const float* __restrict__ c, const float* __restrict__ d,
+                                                      float* __restrict__ x) {
+    extern __shared__ float lds[];
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item;
+        __syncthreads();  // every thread has read `item` before it is rewritten
+        if (it >= ntasks) break;  // uniform: every workgroup leaves once the items are gone
+        const UnevenTask tk = tasks[it];
+        switch (tk.shape) {
+#define TPRH_UNEVEN_CASE(ID, EE, LL)                                                                     \
+    case ID:                                                                                             \
+        run_task<METHOD, MODE, HAS_B, EE, LL>(tk, sys_list, offsets, a, b, c, d, x, lds);                \
+        break;
+            TPRH_UNEVEN_SHAPES(TPRH_UNEVEN_CASE)
+#undef TPRH_UNEVEN_CASE
+            default:
+                break;
+        }
+        __syncthreads();  // the item's LDS rows are read before the next item writes them
+    }
+}
+
+namespace {
+
+template <int METHOD, int MODE, bool HAS_B>
+hipError_t launch_uneven_t(const UnevenLaunch& p, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, p.counter);
+    hipLaunchKernelGGL((uneven_kernel<METHOD, MODE, HAS_B>), dim3(p.grid), dim3(1024), p.lds_bytes, stream, p.tasks,
+                       p.ntasks, p.counter, p.sys_list, p.offsets, p.a, p.b, p.c, p.d, p.x);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_uneven(int method, int mode, const UnevenLaunch& p, hipStream_t stream) {
+    const bool hb = p.b != nullptr;
+    if (method == 0) {
+        if (mode == kExact) return hb ? launch_uneven_t<0, kExact, true>(p, stream) : launch_uneven_t<0, kExact, false>(p, stream);
+        return hb ? launch_uneven_t<0, kFast, true>(p, stream) : launch_uneven_t<0, kFast, false>(p, stream);
+    }
+    if (mode == kExact) return hb ? launch_uneven_t<1, kExact, true>(p, stream) : launch_uneven_t<1, kExact, false>(p, stream);
+    return hb ? launch_uneven_t<1, kFast, true>(p, stream) : launch_uneven_t<1, kFast, false>(p, stream);
+}
+
+hipError_t uneven_occupancy(int method, int mode, size_t lds_bytes, int* blocks_per_cu) {
+    const void* f;
+    if (method == 0) f = mode == kExact ? reinterpret_cast<const void*>(uneven_kernel<0, kExact, true>)
+                                        : reinterpret_cast<const void*>(uneven_kernel<0, kFast, true>);
+    else f = mode == kExact ? reinterpret_cast<const void*>(uneven_kernel<1, kExact, true>)
+                            : reinterpret_cast<const void*>(uneven_kernel<1, kFast, true>);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 1024, lds_bytes);
+}
+
+}  // namespace tprh
