@@ -52,9 +52,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="config 3: systems per round of the split path")
-    ap.add_argument("--config", type=int, default=1, choices=[1, 2, 3],
+    ap.add_argument("--config", type=int, default=1, choices=[1, 2, 3, 4],
                     help="1: the headline batch (default); 2: 20000 systems of uneven N in one persistent launch; "
-                         "3: 256 x N=65536 weakly dominant, precision variants")
+                         "3: 256 x N=65536 weakly dominant, precision variants; 4: the serving loop under Poisson load")
+    ap.add_argument("--rate", type=float, default=2000.0, help="config 4: mean request arrivals per second")
+    ap.add_argument("--seconds", type=float, default=5.0, help="config 4: length of the Poisson run")
     ap.add_argument("--retune", action="store_true",
                     help="config 2: run the autotuner now instead of loading the committed schedule")
     return ap.parse_args()
@@ -388,6 +390,134 @@ def config2(args):
     print(json.dumps(out))
 
 
+def config4(args):
+    """The serving loop under load.  Requests arrive as a Poisson stream
+    (mean args.rate per second); each brings a matrix of N = 2^U{9..15}
+    equations (diagonally dominant generator, generated on the device and
+    resident in HBM) and 1..64 right-hand sides d ~ U(-1,1) (ADI-like
+    reuse of the matrix); the server admits them against a 64 GB budget and
+    batches continuously.  Reported: systems/s (right-hand sides solved per
+    second), p50/p99 latency from submit to completion, and, on the same
+    request mix, the saturated throughput (closed loop) and the plain
+    per-system GPU solve (one launch per right-hand side, the reference's
+    CUDA model)."""
+    import numpy as np
+    import torch
+
+    from tpr_amd import batched as B
+    from tpr_amd.serve import Server
+
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(4044)
+    nreq = max(1, int(args.rate * args.seconds))
+    gaps = rng.exponential(1.0 / args.rate, nreq)
+    arrivals = np.cumsum(gaps)
+    sizes = (1 << rng.integers(9, 16, nreq)).astype(np.int64)
+    nrhs = rng.integers(1, 65, nreq).astype(np.int64)
+    check_every = max(1, nreq // 50)
+    srv = Server(64 << 30, method=args.method, arith=args.arith, max_inflight=4, max_batch_systems=16384)
+
+    def run(order, open_loop, max_outstanding=None):
+        """Serve requests `order`; open loop: at their arrival times."""
+        t_start = srv.now()
+        pending = {}
+        lat, checked = [], []
+        i = 0
+        done = 0
+        while done < len(order):
+            now = srv.now() - t_start
+            while i < len(order) and (not open_loop or arrivals[order[i]] - arrivals[order[0]] <= now) and \
+                    (max_outstanding is None or len(pending) < max_outstanding):
+                k = order[i]
+                m = srv.matrix_generate(int(sizes[k]), seed=4044, sys=k)
+                rid = srv.submit(m, int(nrhs[k]), rhs_seed=4044)
+                pending[rid] = (k, m)
+                i += 1
+            for rid in srv.poll():
+                k, m = pending.pop(rid)
+                inf = srv.info(rid)
+                lat.append(inf["t_done"] - inf["t_submit"])
+                if k % check_every == 0:
+                    checked.append((k, srv.result(rid)[:2].cpu().numpy()))
+                srv.release(rid)
+                srv.matrix_release(m)
+                done += 1
+        return srv.now() - t_start, np.array(lat), checked
+
+    # warm-up: every size class once
+    warm = [int(np.nonzero(sizes == (1 << e))[0][0]) for e in range(9, 16) if (sizes == (1 << e)).any()]
+    run(warm, False)
+    wall, lat, checked = run(list(range(nreq)), True)
+    systems = int(nrhs.sum())
+    eqs = int((sizes * nrhs).sum())
+    st = srv.stats()
+    # saturation: the same mix, closed loop with up to 256 requests outstanding
+    sat_n = min(nreq, 4000)
+    sat_wall, sat_lat, _ = run(list(range(sat_n)), False, max_outstanding=256)
+    sat_systems = int(nrhs[:sat_n].sum())
+    srv.close()
+    # plain per-system solves: one launch per right-hand side (descriptors,
+    # workspaces and inputs prepared beforehand; only the launches are timed)
+    import ctypes
+    from tpr_amd import _lib
+    base_n = min(nreq, 300)
+    prepared = []
+    for k in range(base_n):
+        n = int(sizes[k])
+        t = B.alloc_batch(1, n, with_xt=True)
+        B.generate("dominant", 4044, k, 1, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
+        desc = B.make_desc(t["a"], t["b"], t["c"], t["d"], t["x"], n, min(n, 256), args.method, args.arith)
+        need = B.workspace_bytes(desc)
+        ws = torch.empty(max(need, 1), dtype=torch.uint8, device="cuda")
+        desc.workspace, desc.workspace_bytes = ws.data_ptr(), need
+        prepared.append((desc, int(nrhs[k]), t, ws))
+    torch.cuda.synchronize()
+    solve_ex = _lib.lib().tprh_solve_ex
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    t0 = time.perf_counter()
+    tot_sys = 0
+    for desc, r, _, _ in prepared:
+        ref = ctypes.byref(desc)
+        for j in range(r):
+            if solve_ex(ref, stream) != 0:
+                raise RuntimeError("tprh_solve_ex failed")
+        tot_sys += r
+    torch.cuda.synchronize()
+    base_wall = time.perf_counter() - t0
+    out = {
+        "metric": METRIC, "value": round(systems / wall, 1), "unit": "systems/s", "n_gpus": 1, "steps": nreq,
+        "warmup": len(warm), "ms_per_step": round(1e3 * wall / nreq, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving loop: Poisson arrivals at %.0f req/s for %.1f s (%d requests), N = 2^U{9..15}, "
+                               "1..64 right-hand sides per matrix, matrices generated and resident in HBM, "
+                               "64 GB budget" % (args.rate, args.seconds, nreq),
+                   "method": args.method, "arith": args.arith, "max_inflight": 4},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 4),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 4),
+                       "max": round(1e3 * float(lat.max()), 4)},
+        "served": {"requests": nreq, "systems": systems, "equations": eqs, "wall_s": round(wall, 3),
+                   "batches": st["batches"], "max_batch_systems_seen": st["max_batch_systems_seen"],
+                   "max_inflight_seen": st["max_inflight_seen"], "peak_bytes": st["peak_bytes"]},
+        "saturated": {"requests": sat_n, "systems_per_s": round(sat_systems / sat_wall, 1),
+                      "p50_ms": round(1e3 * float(np.percentile(sat_lat, 50)), 4),
+                      "p99_ms": round(1e3 * float(np.percentile(sat_lat, 99)), 4),
+                      "GBps": round((12 * float((sizes[:sat_n]).sum()) + 8 * float((sizes[:sat_n] * nrhs[:sat_n]).sum()))
+                                    / sat_wall / 1e9, 1)},
+        "per_system_launches": {"requests": base_n, "systems_per_s": round(tot_sys / base_wall, 1)},
+    }
+    if not args.no_cpu:
+        import oracle as O
+        worst = 0.0
+        for k, x in checked:
+            n = int(sizes[k])
+            a, b, c, _, _ = O.gen_system(4044, k, n)
+            for j in range(min(2, int(nrhs[k]))):
+                ref = O.solve_batch(args.method, a, b, c, O.gen_rhs(4044, k, j, n), n, min(n, 256))
+                worst = max(worst, O.rel_l2(x[j], ref))
+        out["accuracy"] = {"requests_checked": len(checked), "max_rel_l2_vs_oracle": worst, "tolerance": 1e-5}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -399,6 +529,9 @@ def main():
         return
     if args.config == 2:
         config2(args)
+        return
+    if args.config == 4:
+        config4(args)
         return
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps

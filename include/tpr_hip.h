@@ -195,6 +195,92 @@ tprh_status tprh_uneven_plan_destroy(tprh_uneven_plan *plan);
 tprh_status tprh_generate(tprh_gen kind, uint64_t seed, uint64_t sys0, int nsys, int n, float *a, float *b,
                           float *c, float *d, float *xt, void *stream);
 
+/* ---- serving: a request loop with continuous batching ----------------------
+ * No counterpart in the reference (its drivers solve one system per call,
+ * src/pm.cpp:140-268); this is the engine a service puts in front of the
+ * solver.  A server owns one device arena of `hbm_budget` bytes that holds
+ * every resident matrix and every admitted request (right-hand sides,
+ * answers, and the split path's workspace for systems above one workgroup).
+ * Matrices stay resident until released; a request solves nrhs right-hand
+ * sides against one matrix.  A worker thread admits queued requests in FIFO
+ * order as arena memory frees up, groups every admitted request into one
+ * persistent launch (plus one split-path launch per request whose n exceeds
+ * a workgroup), and keeps up to `max_inflight` batches on the device at once,
+ * one stream each, so requests that arrive while others run are launched
+ * beside them.  A system's answer depends only on its matrix, right-hand
+ * side and size class (n -> s, E, L), never on the batch it rides in.
+ * All entry points are thread-safe. */
+typedef struct tprh_server tprh_server;
+
+typedef struct tprh_server_config {
+    tprh_method method;     /* TPRH_PTPR or TPRH_TPR */
+    tprh_arith arith;
+    size_t hbm_budget;      /* bytes of the arena (matrices + admitted requests) */
+    int max_inflight;       /* batches on the device at once (>= 1) */
+    int max_batch_systems;  /* right-hand sides per batch (>= 1) */
+    const int *schedule;    /* NULL or 32 (s, E, L) triples as tprh_uneven_plan_create */
+} tprh_server_config;
+
+tprh_status tprh_server_create(tprh_server **out, const tprh_server_config *cfg);
+/* Waits for every queued and in-flight request, then frees everything. */
+tprh_status tprh_server_destroy(tprh_server *srv);
+
+/* Resident matrices of n equations (n a power of two >= 16); fails with
+ * TPRH_ERROR_BUDGET when the arena has no room.  _upload copies a, b, c
+ * (host or device pointers; the copy is complete when the call returns);
+ * _generate fills them on the device with the seeded generator (system id
+ * `sys`, kinds as tprh_generate). */
+tprh_status tprh_server_matrix_upload(tprh_server *srv, int n, const float *a, const float *b, const float *c,
+                                      long long *matrix);
+tprh_status tprh_server_matrix_generate(tprh_server *srv, int n, tprh_gen kind, uint64_t seed, uint64_t sys,
+                                        long long *matrix);
+/* Freed once no queued or in-flight request uses it. */
+tprh_status tprh_server_matrix_release(tprh_server *srv, long long matrix);
+
+/* Queues nrhs right-hand sides against `matrix`.  d: nrhs * n floats (host
+ * or device; copied when the request is launched, so it must stay valid
+ * until then), or NULL to generate right-hand side j as
+ * d ~ U(-1,1) of (rhs_seed, the matrix's generator id, j0 + j).  x: where
+ * the answers go (nrhs * n floats, host or device) when the request
+ * completes, or NULL to keep them in the arena until
+ * tprh_server_request_release (tprh_server_request_info gives the device
+ * address).  TPRH_ERROR_BUDGET: the request needs more than the whole arena. */
+tprh_status tprh_server_submit(tprh_server *srv, long long matrix, int nrhs, const float *d, uint64_t rhs_seed,
+                               int j0, float *x, long long *request);
+
+typedef struct tprh_request_info {
+    int state;           /* 0 queued, 1 on the device, 2 done, 3 failed */
+    int n, nrhs;
+    double t_submit;     /* seconds on the server's clock (tprh_server_now) */
+    double t_launch;
+    double t_done;       /* when the worker saw the batch complete */
+    const float *x;      /* device address of the answers while retained, else NULL */
+} tprh_request_info;
+
+tprh_status tprh_server_request_info(tprh_server *srv, long long request, tprh_request_info *info);
+/* Blocks until the request is done (TPRH_SUCCESS), failed (its error), or
+ * timeout_s passes (TPRH_ERROR_NOT_SUPPORTED with a timeout message). */
+tprh_status tprh_server_wait(tprh_server *srv, long long request, double timeout_s);
+/* Ids of requests completed since the previous call (at most max). */
+tprh_status tprh_server_poll(tprh_server *srv, long long *ids, int max, int *count);
+/* Copies a completed request's retained answers (nrhs * n floats) to dst
+ * (host or device); returns when the copy is done. */
+tprh_status tprh_server_copy_result(tprh_server *srv, long long request, float *dst);
+/* Forgets a completed request and frees its retained answers. */
+tprh_status tprh_server_request_release(tprh_server *srv, long long request);
+double tprh_server_now(const tprh_server *srv);
+
+typedef struct tprh_server_stats {
+    long long submitted, completed, failed;
+    long long systems, equations;  /* right-hand sides and equations solved */
+    long long batches, max_batch_systems_seen;
+    int max_inflight_seen;
+    int queued;
+    size_t bytes_in_use, peak_bytes, budget;
+} tprh_server_stats;
+
+tprh_status tprh_server_stats_get(tprh_server *srv, tprh_server_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,178 @@
+"""GPU tests of the serving loop (tprh_server_*).
+
+* answers: every right-hand side of every request equals the oracle's solve
+  of (generated matrix, generated right-hand side) bit for bit in exact
+  arithmetic, across size classes including the split path (n = 32768);
+* alone vs mixed: a request's answers are identical whether it is served by
+  itself or inside a stream of concurrent requests (fast arithmetic);
+* admission: a budget smaller than the offered work still serves every
+  request (they wait for memory), the arena never exceeds the budget, and a
+  request or matrix that cannot fit is refused with TPRH_ERROR_BUDGET;
+* overlap: several batches are on the device at once.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tpr_amd import _lib
+from tpr_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+SEED = 515
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def bits_equal(x, y):
+    x = np.asarray(x, np.float32)
+    y = np.asarray(y, np.float32)
+    same = x.view(np.uint32) == y.view(np.uint32)
+    return bool(np.all(same | (np.isnan(x) & np.isnan(y))))
+
+
+def oracle_answers(method, sys_id, n, nrhs, j0=0, rhs_seed=SEED, kind=O.GEN_DOMINANT):
+    a, b, c, _, _ = O.gen_system(SEED, sys_id, n, kind)
+    s = min(n, 256)
+    return np.stack([O.solve_batch(method, a, b, c, O.gen_rhs(rhs_seed, sys_id, j0 + j, n), n, s)
+                     for j in range(nrhs)])
+
+
+CASES = [(16, 1), (256, 3), (1024, 8), (4096, 2), (16384, 3), (32768, 2), (512, 5), (2048, 1)]
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_serve_exact_matches_oracle(method):
+    say("serve exact", method)
+    with Server(256 << 20, method=method, arith="exact") as srv:
+        reqs = []
+        for k, (n, nrhs) in enumerate(CASES):
+            m = srv.matrix_generate(n, seed=SEED, sys=100 + k)
+            reqs.append((srv.submit(m, nrhs, rhs_seed=SEED, j0=k), m, n, nrhs, k))
+        for rid, m, n, nrhs, k in reqs:
+            srv.wait(rid)
+            x = srv.result(rid).cpu().numpy()
+            ref = oracle_answers(method, 100 + k, n, nrhs, j0=k)
+            assert bits_equal(x, ref), (n, nrhs)
+            srv.release(rid)
+            srv.matrix_release(m)
+        st = srv.stats()
+        assert st["completed"] == len(CASES) and st["failed"] == 0
+        assert st["bytes_in_use"] == 0
+
+
+def test_serve_upload_and_caller_buffers():
+    """Host-uploaded matrix, right-hand sides from a device tensor, answers
+    copied into a device tensor."""
+    n, nrhs = 2048, 4
+    a, b, c, _, _ = O.gen_system(SEED, 7, n)
+    rhs = np.stack([O.gen_rhs(9, 7, j, n) for j in range(nrhs)]).astype(np.float32)
+    with Server(64 << 20, arith="exact") as srv:
+        m = srv.matrix_upload(a, b, c)
+        d = torch.from_numpy(rhs.ravel().copy()).cuda()
+        x = torch.empty(nrhs * n, device="cuda")
+        rid = srv.submit(m, nrhs, d=d, x=x)
+        srv.wait(rid)
+        assert srv.info(rid)["x_ptr"] is None  # answers went to x, nothing retained
+        ref = np.stack([O.solve_batch("ptpr", a, b, c, rhs[j], n, 256) for j in range(nrhs)])
+        assert bits_equal(x.cpu().numpy().reshape(nrhs, n), ref)
+        srv.release(rid)
+        srv.matrix_release(m)
+        assert srv.stats()["bytes_in_use"] == 0
+
+
+def _serve_all(srv, work, one_by_one):
+    out = {}
+    ids = []
+    for k, (n, nrhs) in enumerate(work):
+        m = srv.matrix_generate(n, seed=SEED, sys=1000 + k)
+        rid = srv.submit(m, nrhs, rhs_seed=SEED)
+        ids.append((k, rid, m))
+        if one_by_one:
+            srv.wait(rid)
+    for k, rid, m in ids:
+        srv.wait(rid)
+        out[k] = srv.result(rid).cpu().numpy()
+        srv.release(rid)
+        srv.matrix_release(m)
+    return out
+
+
+def test_serve_alone_equals_mixed():
+    rng = np.random.default_rng(5)
+    work = [(int(1 << rng.integers(9, 16)), int(rng.integers(1, 65))) for _ in range(40)]
+    say("alone vs mixed", len(work))
+    with Server(1 << 30, max_inflight=4, max_batch_systems=96) as srv:
+        alone = _serve_all(srv, work, True)
+        mixed = _serve_all(srv, work, False)
+        st = srv.stats()
+    say(st)
+    assert st["max_inflight_seen"] >= 2
+    worst = 0.0
+    for k, (n, nrhs) in enumerate(work):
+        assert bits_equal(alone[k], mixed[k]), k
+        if k % 8 == 0:
+            ref = oracle_answers("ptpr", 1000 + k, n, min(nrhs, 4))
+            worst = max(worst, max(O.rel_l2(alone[k][j], ref[j]) for j in range(min(nrhs, 4))))
+    assert worst <= FAST_TOL
+
+
+def test_serve_budget_admission():
+    n, nrhs = 4096, 16
+    req_bytes = 2 * nrhs * n * 4          # right-hand sides + answers
+    budget = 6 * req_bytes                # room for a few requests at a time
+    with Server(budget, max_inflight=4) as srv:
+        mats = [srv.matrix_generate(n, seed=SEED, sys=2000 + k) for k in range(3)]
+        xs = [torch.empty(nrhs * n, device="cuda") for _ in range(24)]
+        rids = [srv.submit(mats[k % 3], nrhs, x=xs[k], rhs_seed=SEED, j0=k) for k in range(24)]
+        for r in rids:
+            srv.wait(r)
+        st = srv.stats()
+        assert st["completed"] == 24 and st["failed"] == 0
+        assert st["peak_bytes"] <= budget
+        for k in (0, 11, 23):
+            ref = oracle_answers("ptpr", 2000 + k % 3, n, 1, j0=k)
+            got = xs[k].cpu().numpy()[:n]
+            assert O.rel_l2(got, ref[0]) <= FAST_TOL
+        # larger than the whole budget: refused at submit
+        with pytest.raises(_lib.TprError) as ei:
+            srv.submit(mats[0], 200)
+        assert ei.value.status == _lib.ERROR_BUDGET
+        for r in rids:
+            srv.release(r)
+        # matrices fill the arena: the next one is refused
+        big = []
+        with pytest.raises(_lib.TprError) as ei:
+            for k in range(1000):
+                big.append(srv.matrix_generate(16384, sys=k))
+        assert ei.value.status == _lib.ERROR_BUDGET
+        for m in big + mats:
+            srv.matrix_release(m)
+        assert srv.stats()["bytes_in_use"] == 0
+
+
+def test_serve_errors():
+    with Server(16 << 20) as srv:
+        with pytest.raises(_lib.TprError):
+            srv.matrix_generate(1000)
+        m = srv.matrix_generate(1024)
+        with pytest.raises(_lib.TprError):
+            srv.submit(m, 0)
+        with pytest.raises(_lib.TprError):
+            srv.submit(m + 99, 1)
+        rid = srv.submit(m, 1)
+        srv.wait(rid)
+        assert srv.info(rid)["state"] == "done"
+        srv.release(rid)
+        with pytest.raises(_lib.TprError):
+            srv.release(rid)
+        srv.matrix_release(m)
+        with pytest.raises(_lib.TprError):
+            srv.matrix_release(m)

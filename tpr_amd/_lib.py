@@ -73,6 +73,48 @@ SIGNATURES.update({
     "tprh_uneven_plan_destroy": (_i, [_vp]),
 })
 
+
+
+class ServerConfig(ctypes.Structure):
+    """tprh_server_config of include/tpr_hip.h."""
+    _fields_ = [("method", ctypes.c_int), ("arith", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("max_inflight", ctypes.c_int), ("max_batch_systems", ctypes.c_int),
+                ("schedule", ctypes.c_void_p)]
+
+
+class RequestInfo(ctypes.Structure):
+    """tprh_request_info of include/tpr_hip.h."""
+    _fields_ = [("state", ctypes.c_int), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_launch", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("x", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    """tprh_server_stats of include/tpr_hip.h."""
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+                ("systems", ctypes.c_longlong), ("equations", ctypes.c_longlong), ("batches", ctypes.c_longlong),
+                ("max_batch_systems_seen", ctypes.c_longlong), ("max_inflight_seen", ctypes.c_int),
+                ("queued", ctypes.c_int), ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t),
+                ("budget", ctypes.c_size_t)]
+
+
+_ll = ctypes.c_longlong
+SIGNATURES.update({
+    "tprh_server_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(ServerConfig)]),
+    "tprh_server_destroy": (_i, [_vp]),
+    "tprh_server_matrix_upload": (_i, [_vp, _i, _vp, _vp, _vp, ctypes.POINTER(_ll)]),
+    "tprh_server_matrix_generate": (_i, [_vp, _i, _i, _u64, _u64, ctypes.POINTER(_ll)]),
+    "tprh_server_matrix_release": (_i, [_vp, _ll]),
+    "tprh_server_submit": (_i, [_vp, _ll, _i, _vp, _u64, _i, _vp, ctypes.POINTER(_ll)]),
+    "tprh_server_request_info": (_i, [_vp, _ll, ctypes.POINTER(RequestInfo)]),
+    "tprh_server_wait": (_i, [_vp, _ll, ctypes.c_double]),
+    "tprh_server_poll": (_i, [_vp, ctypes.POINTER(_ll), _i, ctypes.POINTER(_i)]),
+    "tprh_server_copy_result": (_i, [_vp, _ll, _vp]),
+    "tprh_server_request_release": (_i, [_vp, _ll]),
+    "tprh_server_now": (ctypes.c_double, [_vp]),
+    "tprh_server_stats_get": (_i, [_vp, ctypes.POINTER(ServerStats)]),
+})
+
 STORE_F32, STORE_F16 = 0, 1
 
 _lib = None

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
     const size_t x0 = valid ? (size_t)sys * (size_t)n : 0;
     const size_t off = x0 + (size_t)t * E;
     RawPart<E> raw;
-    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, off, raw);
     float xo[E];
     if constexpr (REFINE) {
         // every lane of the system loads x here, before the barriers of the

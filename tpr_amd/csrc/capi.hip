@@ -267,6 +267,37 @@ int ilog2i(int v) {
 
 }  // namespace
 
+// Internal entry points for the other host-side translation units.
+namespace tprh {
+
+tprh_status set_error(tprh_status st, const std::string& msg) { return fail(st, msg); }
+tprh_status set_hip_error(hipError_t e, const char* what) { return fail_hip(e, what); }
+
+// Slice size and shape of the size class n under a 32-row (s, E, L) schedule
+// (NULL or a zero row: s = min(n, 256), default shape).  *split: the class
+// needs the multi-workgroup path.
+tprh_status class_shape(int n, const int* schedule, int* s_out, int* E_out, int* L_out, bool allow_split,
+                        bool* split) {
+    if (n < 16 || !is_pow2(n)) return fail(TPRH_ERROR_INVALID_VALUE, "system sizes must be powers of two >= 16");
+    const int lg = __builtin_ctz((unsigned)n);
+    int s = n < 256 ? n : 256, E = 0, L = 0;
+    bool expl = false;
+    if (schedule && schedule[3 * lg] != 0) {
+        s = schedule[3 * lg];
+        E = schedule[3 * lg + 1];
+        L = schedule[3 * lg + 2];
+        expl = E != 0 || L != 0;
+    }
+    tprh_status st = check_ns(n, s, 1, &E, &L, expl, allow_split, split);
+    if (st != TPRH_SUCCESS) return st;
+    *s_out = s;
+    *E_out = E;
+    *L_out = L;
+    return TPRH_SUCCESS;
+}
+
+}  // namespace tprh
+
 struct tprh_uneven_plan {
     tprh_method method;
     tprh_arith arith;
@@ -464,17 +495,9 @@ tprh_status tprh_uneven_plan_create(tprh_uneven_plan** out, tprh_method method, 
         if (nk < 16 || !is_pow2(nk)) return fail(TPRH_ERROR_INVALID_VALUE, "system sizes must be powers of two >= 16");
         offsets[k + 1] = offsets[k] + nk;
         if (!classes.count(nk)) {
-            const int lg = __builtin_ctz((unsigned)nk);
-            int s = nk < 256 ? nk : 256, E = 0, L = 0;
-            bool expl = false;
-            if (schedule && schedule[3 * lg] != 0) {
-                s = schedule[3 * lg];
-                E = schedule[3 * lg + 1];
-                L = schedule[3 * lg + 2];
-                expl = E != 0 || L != 0;
-            }
+            int s = 0, E = 0, L = 0;
             bool split = false;
-            tprh_status st = check_ns(nk, s, 1, &E, &L, expl, false, &split);
+            tprh_status st = tprh::class_shape(nk, schedule, &s, &E, &L, false, &split);
             if (st != TPRH_SUCCESS) return st;
             if (tprh::uneven_shape_id(E, L) < 0)
                 return fail(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the uneven-batch kernel");

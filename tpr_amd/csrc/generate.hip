@@ -117,6 +117,33 @@ __global__ void generate_rhs_kernel(uint64_t seed, uint64_t sys, int j0, int nrh
     }
 }
 
+// The coefficients of system `sys` only, packed [a | b | c] (n floats each):
+// the same values generate_one writes.
+__global__ void generate_coef_kernel(int kind, uint64_t seed, uint64_t sys, int n, float* __restrict__ abc) {
+    const uint64_t key = mix64(seed + kGolden * (sys + 1));
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float ai, bi, ci;
+        if (kind == 2 || kind == 3) {
+            ai = i == 0 ? 0.0f : (float)(-1.0 / 6.0);
+            ci = i == n - 1 ? 0.0f : (float)(-1.0 / 6.0);
+            bi = 1.0f;
+        } else {
+            gen_coef(key, i, n, kind, ai, bi, ci);
+        }
+        abc[i] = ai;
+        abc[n + i] = bi;
+        abc[2 * (size_t)n + i] = ci;
+    }
+}
+
+hipError_t launch_generate_coef(int kind, uint64_t seed, uint64_t sys, int n, float* abc, hipStream_t stream) {
+    int blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(generate_coef_kernel, dim3(blocks), dim3(256), 0, stream, kind, seed, sys, n, abc);
+    return hipGetLastError();
+}
+
 hipError_t launch_generate(int kind, uint64_t seed, uint64_t sys0, int nsys, int n, float* a, float* b, float* c,
                            float* d, float* xt, hipStream_t stream) {
     const size_t total = (size_t)nsys * (size_t)n;

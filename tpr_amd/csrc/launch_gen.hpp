@@ -12,6 +12,7 @@ hipError_t launch_generate_uneven(int kind, uint64_t seed, uint64_t sys0, int ns
                                   const long long* offsets, float* a, float* b, float* c, float* d, float* xt,
                                   hipStream_t stream);
 hipError_t launch_generate_rhs(uint64_t seed, uint64_t sys, int j0, int nrhs, int n, float* d, hipStream_t stream);
+hipError_t launch_generate_coef(int kind, uint64_t seed, uint64_t sys, int n, float* abc, hipStream_t stream);
 
 // Largest n the single-workgroup PCR / CR kernels accept.
 constexpr int kSingleMaxN = 4096;

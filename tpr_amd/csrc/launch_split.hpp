@@ -31,6 +31,7 @@ struct SplitArgs {
     float* x;
     int n, nsys, chunk;
     bool f16, refine;
+    bool shared_coef;  // every system uses the one matrix at a, b, c (several right-hand sides)
     // workspace, for one round of `chunk` systems
     float* rows;  // chunk * m * kSplitRowWords: boundary rows (and x) per slice
     float* xed;   // chunk * m: x at the end of every slice

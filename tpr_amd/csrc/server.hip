@@ -1,0 +1,777 @@
+// server.hip -- the serving loop behind tprh_server_* (include/tpr_hip.h).
+//
+// Host code only.  One device arena (a single hipMalloc of the budget) holds
+// resident matrices and admitted requests; a first-fit allocator with
+// coalescing hands out 256-byte aligned blocks, so the loop itself never
+// calls hipMalloc.  A worker thread runs
+//
+//   1. completion: every in-flight batch whose event has fired retires its
+//      requests (answers copied out or retained, arena blocks freed);
+//   2. admission + launch: while a batch slot is free, queued requests are
+//      taken in FIFO order as long as the arena has room for them and the
+//      batch has room for their right-hand sides; the batch's work-item
+//      table is built on the host, copied through a pinned staging buffer,
+//      and one persistent serve kernel (plus a split-path launch for each
+//      request above one workgroup) runs on the slot's own stream;
+//   3. idle: sleeps on a condition variable when nothing is queued or in
+//      flight, otherwise polls the in-flight events.
+//
+// Batches on different slots overlap on the device.  Every stream has the
+// default priority.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/tpr_hip.h"
+#include "launch.hpp"
+#include "launch_gen.hpp"
+#include "launch_serve.hpp"
+#include "launch_split.hpp"
+
+namespace tprh {
+tprh_status set_error(tprh_status st, const std::string& msg);
+tprh_status set_hip_error(hipError_t e, const char* what);
+tprh_status class_shape(int n, const int* schedule, int* s, int* E, int* L, bool allow_split, bool* split);
+}  // namespace tprh
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over [0, cap) bytes; free blocks coalesce.
+class Arena {
+public:
+    explicit Arena(size_t cap) : cap_(cap) {
+        if (cap > 0) free_[0] = cap;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = align_up(bytes == 0 ? 1 : bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes;
+            const size_t at = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align_up(bytes == 0 ? 1 : bytes);
+        used_ -= bytes;
+        auto next = free_.lower_bound(off);
+        size_t start = off, len = bytes;
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                start = prev->first;
+                len += prev->second;
+                free_.erase(prev);
+            }
+        }
+        if (next != free_.end() && next->first == off + bytes) {
+            len += next->second;
+            free_.erase(next);
+        }
+        free_[start] = len;
+    }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+    size_t cap() const { return cap_; }
+
+private:
+    size_t cap_, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+};
+
+struct Matrix {
+    int n = 0;
+    size_t off = 0, bytes = 0;  // packed a | b | c in the arena
+    uint64_t sys = 0;           // generator id (right-hand sides generated for it use it)
+    int refs = 0;               // queued or in-flight requests
+    bool released = false;
+    bool ready_known = false;   // the ready event has been seen complete
+    hipEvent_t ready = nullptr;
+};
+
+enum { kQueued = 0, kOnDevice = 1, kDone = 2, kFailed = 3 };
+
+struct Request {
+    long long mat = 0;
+    int n = 0, nrhs = 0;
+    int s = 0, E = 0, L = 0;
+    bool split = false;
+    const float* d_src = nullptr;
+    float* x_dst = nullptr;
+    uint64_t rhs_seed = 0;
+    int j0 = 0;
+    size_t off = 0, bytes = 0;  // arena block: d | x | split workspace
+    size_t x_off = 0, ws_off = 0;
+    bool holds_block = false;
+    int state = kQueued;
+    tprh_status status = TPRH_SUCCESS;
+    double t_submit = 0, t_launch = 0, t_done = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    void* host_stage = nullptr;  // pinned: tasks | entries
+    void* dev_table = nullptr;   // device: tasks | entries | counter
+    bool busy = false;
+    std::vector<long long> reqs;
+    long long systems = 0, equations = 0;
+};
+
+}  // namespace
+
+struct tprh_server {
+    tprh_server_config cfg{};
+    int schedule[96] = {};
+    bool have_schedule = false;
+    int method = 0, mode = 0;
+    int device = 0;
+    char* base = nullptr;  // arena base (device)
+    Arena arena{0};
+    std::chrono::steady_clock::time_point t0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<long long> queue;
+    std::unordered_map<long long, Request> reqs;
+    std::unordered_map<long long, Matrix> mats;
+    std::vector<long long> completed;
+    long long next_req = 1, next_mat = 1;
+    std::vector<Slot> slots;
+    hipStream_t upload = nullptr;
+    int grid = 1;
+    size_t lds_bytes = 0;
+    size_t table_tasks_bytes = 0, table_bytes = 0;
+    std::thread worker;
+    bool stop = false;
+    std::string worker_error;
+
+    tprh_server_stats st{};
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+// LDS of the serve kernel: a work item holds 1024 / (m L) systems of 13 m
+// floats each, at most 13 * 1024 / 16 floats for any shape of the menu.
+constexpr size_t kServeLds = sizeof(float) * 13 * 1024 / 16;
+
+void free_matrix_locked(tprh_server* sv, long long id) {
+    auto it = sv->mats.find(id);
+    if (it == sv->mats.end()) return;
+    Matrix& m = it->second;
+    if (m.ready) {
+        (void)hipEventSynchronize(m.ready);  // its generation or upload is finished before the block is reused
+        (void)hipEventDestroy(m.ready);
+    }
+    sv->arena.release(m.off, m.bytes);
+    sv->mats.erase(it);
+}
+
+void drop_request_ref_locked(tprh_server* sv, Request& r) {
+    auto it = sv->mats.find(r.mat);
+    if (it == sv->mats.end()) return;
+    if (--it->second.refs == 0 && it->second.released) free_matrix_locked(sv, r.mat);
+}
+
+// Retires the batch of a slot whose event has fired (or failed).
+void retire_locked(tprh_server* sv, Slot& sl, hipError_t e) {
+    const double t = sv->now();
+    for (long long id : sl.reqs) {
+        auto it = sv->reqs.find(id);
+        if (it == sv->reqs.end()) continue;
+        Request& r = it->second;
+        r.t_done = t;
+        if (e != hipSuccess) {
+            r.state = kFailed;
+            r.status = TPRH_ERROR_HIP;
+            sv->st.failed++;
+        } else {
+            r.state = kDone;
+            sv->st.completed++;
+        }
+        // the right-hand sides and workspace always go; the answers stay only
+        // when the caller asked to keep them in the arena
+        const bool keep_x = r.x_dst == nullptr && e == hipSuccess;
+        if (r.holds_block) {
+            if (keep_x) {
+                const size_t xb = sizeof(float) * (size_t)r.nrhs * r.n;
+                // keep [x_off, x_off + xb): free the parts before and after it
+                const size_t head = r.x_off - r.off;
+                const size_t tail_off = r.x_off + align_up(xb);
+                const size_t tail = r.off + r.bytes - tail_off;
+                if (head) sv->arena.release(r.off, head);
+                if (tail) sv->arena.release(tail_off, tail);
+                r.off = r.x_off;
+                r.bytes = align_up(xb);
+            } else {
+                sv->arena.release(r.off, r.bytes);
+                r.holds_block = false;
+            }
+        }
+        drop_request_ref_locked(sv, r);
+        sv->completed.push_back(id);
+    }
+    if (e == hipSuccess) {
+        sv->st.systems += sl.systems;
+        sv->st.equations += sl.equations;
+    } else {
+        sv->worker_error = std::string("batch failed: ") + hipGetErrorString(e);
+    }
+    sl.reqs.clear();
+    sl.busy = false;
+}
+
+struct ClassKey {
+    int n;
+    bool operator<(const ClassKey& o) const { return n > o.n; }  // largest systems first
+};
+
+// Builds and launches one batch on slot `sl` from the requests in `ids`
+// (already holding their arena blocks).  Returns the first HIP error.
+hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long long>& ids) {
+    float* basef = reinterpret_cast<float*>(sv->base);
+    const size_t fsz = sizeof(float);
+    // work items: group the right-hand sides of the non-split requests by class
+    std::map<ClassKey, std::vector<tprh::ServeSys>> groups;
+    std::map<int, std::pair<int, int>> shape_of;  // n -> (E, L)
+    hipError_t e = hipSuccess;
+    for (long long id : ids) {
+        Request& r = sv->reqs[id];
+        Matrix& m = sv->mats[r.mat];
+        if (!m.ready_known) {
+            hipError_t q = hipEventQuery(m.ready);
+            if (q == hipSuccess) m.ready_known = true;
+            else if ((e = hipStreamWaitEvent(sl.stream, m.ready, 0)) != hipSuccess) return e;
+        }
+        const size_t nn = (size_t)r.n;
+        float* d = basef + r.off / fsz;
+        if (r.d_src) {
+            e = hipMemcpyAsync(d, r.d_src, fsz * nn * r.nrhs, hipMemcpyDefault, sl.stream);
+        } else {
+            e = tprh::launch_generate_rhs(r.rhs_seed, m.sys, r.j0, r.nrhs, r.n, d, sl.stream);
+        }
+        if (e != hipSuccess) return e;
+        if (r.split) continue;
+        auto& g = groups[ClassKey{r.n}];
+        shape_of[r.n] = {r.E, r.L};
+        for (int j = 0; j < r.nrhs; j++) {
+            tprh::ServeSys ss;
+            ss.coef = (long long)(m.off / fsz);
+            ss.rhs = (long long)(r.off / fsz + (size_t)j * nn);
+            ss.x = (long long)(r.x_off / fsz + (size_t)j * nn);
+            g.push_back(ss);
+        }
+    }
+    std::vector<tprh::UnevenTask> tasks;
+    std::vector<tprh::ServeSys> entries;
+    for (auto& kv : groups) {
+        const int n = kv.first.n;
+        const int E = shape_of[n].first, L = shape_of[n].second;
+        const int m = n / (E * L);
+        const int T = m * L;
+        const int per = 1024 / T;
+        const std::vector<tprh::ServeSys>& v = kv.second;
+        for (size_t i = 0; i < v.size(); i += per) {
+            tprh::UnevenTask tk;
+            tk.shape = tprh::uneven_shape_id(E, L);
+            tk.m = m;
+            tk.lg_lanes = __builtin_ctz((unsigned)T);
+            tk.first = (int)entries.size();
+            tk.count = (int)std::min<size_t>(per, v.size() - i);
+            for (int q = 0; q < tk.count; q++) entries.push_back(v[i + q]);
+            tasks.push_back(tk);
+        }
+    }
+    if (!tasks.empty()) {
+        const size_t tb = sizeof(tprh::UnevenTask) * tasks.size();
+        const size_t eb = sizeof(tprh::ServeSys) * entries.size();
+        char* hs = static_cast<char*>(sl.host_stage);
+        std::memcpy(hs, tasks.data(), tb);
+        std::memcpy(hs + sv->table_tasks_bytes, entries.data(), eb);
+        char* dt = static_cast<char*>(sl.dev_table);
+        if ((e = hipMemcpyAsync(dt, hs, tb, hipMemcpyHostToDevice, sl.stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(dt + sv->table_tasks_bytes, hs + sv->table_tasks_bytes, eb, hipMemcpyHostToDevice,
+                                sl.stream)) != hipSuccess)
+            return e;
+        tprh::ServeLaunch p;
+        p.tasks = reinterpret_cast<const tprh::UnevenTask*>(dt);
+        p.ntasks = (int)tasks.size();
+        p.counter = reinterpret_cast<int*>(dt + sv->table_bytes);
+        p.sys = reinterpret_cast<const tprh::ServeSys*>(dt + sv->table_tasks_bytes);
+        p.base = basef;
+        p.grid = std::max(1, std::min(sv->grid, p.ntasks));
+        p.lds_bytes = sv->lds_bytes;
+        if ((e = tprh::launch_serve(sv->method, sv->mode, p, sl.stream)) != hipSuccess) return e;
+    }
+    // requests above one workgroup: the split path, all right-hand sides on
+    // the one resident matrix
+    for (long long id : ids) {
+        Request& r = sv->reqs[id];
+        if (!r.split) continue;
+        Matrix& m = sv->mats[r.mat];
+        tprh::SplitArgs p{};
+        const float* a = basef + m.off / fsz;
+        p.a = a;
+        p.b = a + r.n;
+        p.c = a + 2 * (size_t)r.n;
+        p.d = basef + r.off / fsz;
+        p.x = basef + r.x_off / fsz;
+        p.n = r.n;
+        p.nsys = r.nrhs;
+        p.chunk = r.nrhs;
+        p.shared_coef = true;
+        tprh::split_carve(sv->base + r.ws_off, r.n / r.s, r.nrhs, &p);
+        if ((e = tprh::launch_split(sv->method, sv->mode, p, r.E, r.L, sl.stream)) != hipSuccess) return e;
+    }
+    for (long long id : ids) {
+        Request& r = sv->reqs[id];
+        if (r.x_dst &&
+            (e = hipMemcpyAsync(r.x_dst, basef + r.x_off / fsz, fsz * (size_t)r.n * r.nrhs, hipMemcpyDefault,
+                                sl.stream)) != hipSuccess)
+            return e;
+    }
+    return hipEventRecord(sl.done, sl.stream);
+}
+
+void worker_main(tprh_server* sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+        // 1. completions
+        int inflight = 0;
+        for (Slot& sl : sv->slots) {
+            if (!sl.busy) continue;
+            hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) {
+                inflight++;
+                continue;
+            }
+            retire_locked(sv, sl, q);
+            sv->cv_done.notify_all();
+        }
+        // 2. admission and launch
+        bool launched = false;
+        for (Slot& sl : sv->slots) {
+            if (sl.busy || sv->queue.empty()) continue;
+            std::vector<long long> batch;
+            long long systems = 0, equations = 0;
+            while (!sv->queue.empty()) {
+                const long long id = sv->queue.front();
+                Request& r = sv->reqs[id];
+                if (!batch.empty() && systems + r.nrhs > sv->cfg.max_batch_systems) break;
+                size_t off;
+                if (!sv->arena.alloc(r.bytes, &off)) break;  // FIFO: wait for memory
+                r.off = off;
+                r.holds_block = true;
+                const size_t xb = sizeof(float) * (size_t)r.nrhs * r.n;
+                r.x_off = off + align_up(xb);
+                r.ws_off = r.x_off + align_up(xb);
+                sv->queue.pop_front();
+                batch.push_back(id);
+                systems += r.nrhs;
+                equations += (long long)r.nrhs * r.n;
+            }
+            if (batch.empty()) break;
+            const double t = sv->now();
+            for (long long id : batch) {
+                Request& r = sv->reqs[id];
+                r.state = kOnDevice;
+                r.t_launch = t;
+            }
+            sl.reqs = batch;
+            sl.systems = systems;
+            sl.equations = equations;
+            sl.busy = true;
+            hipError_t e = launch_batch_locked(sv, sl, batch);
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(sl.stream);
+                retire_locked(sv, sl, e);
+                sv->cv_done.notify_all();
+                continue;
+            }
+            sv->st.batches++;
+            sv->st.max_batch_systems_seen = std::max(sv->st.max_batch_systems_seen, systems);
+            inflight++;
+            launched = true;
+            sv->st.max_inflight_seen = std::max(sv->st.max_inflight_seen, inflight);
+        }
+        if (sv->stop && inflight == 0 && !launched && !sv->queue.empty()) {
+            // shutting down with requests that can never be admitted (the
+            // memory they wait for is held by matrices or retained answers)
+            for (long long id : sv->queue) {
+                Request& r = sv->reqs[id];
+                r.state = kFailed;
+                r.status = TPRH_ERROR_BUDGET;
+                r.t_done = sv->now();
+                drop_request_ref_locked(sv, r);
+                sv->completed.push_back(id);
+                sv->st.failed++;
+            }
+            sv->queue.clear();
+            sv->worker_error = "server destroyed before the request could be admitted";
+            sv->cv_done.notify_all();
+        }
+        sv->st.queued = (int)sv->queue.size();
+        if (sv->stop && sv->queue.empty() && inflight == 0) break;
+        // 3. idle or poll
+        if (inflight == 0 && !launched) {
+            sv->cv_work.wait_for(lk, std::chrono::milliseconds(50));
+        } else if (!launched) {
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        }
+    }
+}
+
+tprh_status validate_n(tprh_server* sv, int n, int* s, int* E, int* L, bool* split) {
+    return tprh::class_shape(n, sv->have_schedule ? sv->schedule : nullptr, s, E, L, true, split);
+}
+
+}  // namespace
+
+extern "C" {
+
+tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg) {
+    if (!out || !cfg) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    *out = nullptr;
+    if (cfg->method != TPRH_PTPR && cfg->method != TPRH_TPR)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "method must be TPRH_PTPR or TPRH_TPR");
+    if (cfg->arith != TPRH_ARITH_FAST && cfg->arith != TPRH_ARITH_EXACT)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "bad arith");
+    if (cfg->max_inflight < 1 || cfg->max_inflight > 16)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "max_inflight must be 1..16");
+    if (cfg->max_batch_systems < 1 || cfg->max_batch_systems > (1 << 20))
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "max_batch_systems must be 1..2^20");
+    if (cfg->hbm_budget < kAlign) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "hbm_budget too small");
+    if (cfg->schedule) {
+        // every row must name a shape the serve kernel has (or be zero)
+        for (int lg = 4; lg < 32; lg++) {
+            if (cfg->schedule[3 * lg] == 0) continue;
+            int s, E, L;
+            bool split;
+            tprh_status st = tprh::class_shape(1 << lg, cfg->schedule, &s, &E, &L, true, &split);
+            if (st != TPRH_SUCCESS) return st;
+            if (!split && tprh::uneven_shape_id(E, L) < 0)
+                return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the serve kernel");
+        }
+    }
+    tprh_server* sv = new (std::nothrow) tprh_server();
+    if (!sv) return tprh::set_error(TPRH_ERROR_OUT_OF_MEMORY, "server allocation");
+    sv->cfg = *cfg;
+    if (cfg->schedule) {
+        std::copy(cfg->schedule, cfg->schedule + 96, sv->schedule);
+        sv->have_schedule = true;
+    }
+    sv->cfg.schedule = nullptr;
+    sv->method = cfg->method == TPRH_PTPR ? 0 : 1;
+    sv->mode = cfg->arith == TPRH_ARITH_EXACT ? 1 : 0;
+    sv->t0 = std::chrono::steady_clock::now();
+    auto bail = [&](hipError_t e, const char* what) {
+        for (Slot& sl : sv->slots) {
+            if (sl.stream) (void)hipStreamDestroy(sl.stream);
+            if (sl.done) (void)hipEventDestroy(sl.done);
+            if (sl.host_stage) (void)hipHostFree(sl.host_stage);
+            if (sl.dev_table) (void)hipFree(sl.dev_table);
+        }
+        if (sv->upload) (void)hipStreamDestroy(sv->upload);
+        if (sv->base) (void)hipFree(sv->base);
+        delete sv;
+        return tprh::set_hip_error(e, what);
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&sv->device)) != hipSuccess) return bail(e, "hipGetDevice");
+    if ((e = hipMalloc(reinterpret_cast<void**>(&sv->base), cfg->hbm_budget)) != hipSuccess)
+        return bail(e, "hipMalloc(server arena)");
+    sv->arena = Arena(cfg->hbm_budget);
+    if ((e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    // per-slot work-item tables: at most one item per right-hand side
+    sv->table_tasks_bytes = align_up(sizeof(tprh::UnevenTask) * (size_t)cfg->max_batch_systems);
+    sv->table_bytes = sv->table_tasks_bytes + align_up(sizeof(tprh::ServeSys) * (size_t)cfg->max_batch_systems);
+    sv->slots.resize(cfg->max_inflight);
+    for (Slot& sl : sv->slots) {
+        if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
+            return bail(e, "hipStreamCreate");
+        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
+            return bail(e, "hipEventCreate");
+        if ((e = hipHostMalloc(&sl.host_stage, sv->table_bytes)) != hipSuccess) return bail(e, "hipHostMalloc");
+        if ((e = hipMalloc(&sl.dev_table, sv->table_bytes + kAlign)) != hipSuccess) return bail(e, "hipMalloc");
+    }
+    int cus = 0, occ = 0;
+    sv->lds_bytes = kServeLds;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, sv->device)) != hipSuccess ||
+        (e = tprh::serve_occupancy(sv->method, sv->mode, sv->lds_bytes, &occ)) != hipSuccess)
+        return bail(e, "device query");
+    sv->grid = std::max(1, cus * std::max(1, occ));
+    sv->st.budget = cfg->hbm_budget;
+    try {
+        sv->worker = std::thread(worker_main, sv);
+    } catch (...) {
+        return bail(hipErrorUnknown, "worker thread");
+    }
+    *out = sv;
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_destroy(tprh_server* sv) {
+    if (!sv) return TPRH_SUCCESS;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipStreamSynchronize(sv->upload);
+    for (auto& kv : sv->mats)
+        if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
+    for (Slot& sl : sv->slots) {
+        (void)hipStreamDestroy(sl.stream);
+        (void)hipEventDestroy(sl.done);
+        (void)hipHostFree(sl.host_stage);
+        (void)hipFree(sl.dev_table);
+    }
+    (void)hipStreamDestroy(sv->upload);
+    (void)hipFree(sv->base);
+    delete sv;
+    return TPRH_SUCCESS;
+}
+
+static tprh_status new_matrix(tprh_server* sv, int n, long long* id, Matrix** out) {
+    if (!id) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    int s, E, L;
+    bool split;
+    tprh_status st = validate_n(sv, n, &s, &E, &L, &split);
+    if (st != TPRH_SUCCESS) return st;
+    const size_t bytes = sizeof(float) * 3 * (size_t)n;
+    size_t off;
+    if (!sv->arena.alloc(bytes, &off))
+        return tprh::set_error(TPRH_ERROR_BUDGET, "no room for the matrix in the server's memory budget");
+    hipEvent_t ev;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        sv->arena.release(off, bytes);
+        return tprh::set_hip_error(e, "hipEventCreate");
+    }
+    const long long mid = sv->next_mat++;
+    Matrix& m = sv->mats[mid];
+    m.n = n;
+    m.off = off;
+    m.bytes = bytes;
+    m.ready = ev;
+    *id = mid;
+    *out = &m;
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_matrix_upload(tprh_server* sv, int n, const float* a, const float* b, const float* c,
+                                      long long* matrix) {
+    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    if (!a || !b || !c) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null array pointer");
+    std::lock_guard<std::mutex> g(sv->mu);
+    Matrix* m;
+    tprh_status st = new_matrix(sv, n, matrix, &m);
+    if (st != TPRH_SUCCESS) return st;
+    float* dst = reinterpret_cast<float*>(sv->base + m->off);
+    const size_t nb = sizeof(float) * (size_t)n;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dst, a, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
+        (e = hipMemcpyAsync(dst + n, b, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
+        (e = hipMemcpyAsync(dst + 2 * (size_t)n, c, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
+        (e = hipEventRecord(m->ready, sv->upload)) != hipSuccess || (e = hipEventSynchronize(m->ready)) != hipSuccess) {
+        free_matrix_locked(sv, *matrix);
+        return tprh::set_hip_error(e, "matrix upload");
+    }
+    m->ready_known = true;
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_matrix_generate(tprh_server* sv, int n, tprh_gen kind, uint64_t seed, uint64_t sys,
+                                        long long* matrix) {
+    if (kind < TPRH_GEN_DOMINANT || kind > TPRH_GEN_RANDOM_RHS)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "bad kind");
+    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    Matrix* m;
+    tprh_status st = new_matrix(sv, n, matrix, &m);
+    if (st != TPRH_SUCCESS) return st;
+    m->sys = sys;
+    float* dst = reinterpret_cast<float*>(sv->base + m->off);
+    hipError_t e;
+    if ((e = tprh::launch_generate_coef((int)kind, seed, sys, n, dst, sv->upload)) != hipSuccess ||
+        (e = hipEventRecord(m->ready, sv->upload)) != hipSuccess) {
+        free_matrix_locked(sv, *matrix);
+        return tprh::set_hip_error(e, "matrix generation");
+    }
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_matrix_release(tprh_server* sv, long long matrix) {
+    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->mats.find(matrix);
+    if (it == sv->mats.end() || it->second.released)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown matrix");
+    it->second.released = true;
+    if (it->second.refs == 0) {
+        free_matrix_locked(sv, matrix);
+        sv->cv_work.notify_one();  // queued requests may fit now
+    }
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_submit(tprh_server* sv, long long matrix, int nrhs, const float* d, uint64_t rhs_seed, int j0,
+                               float* x, long long* request) {
+    if (!sv || !request) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->mats.find(matrix);
+    if (it == sv->mats.end() || it->second.released)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown matrix");
+    if (nrhs < 1 || nrhs > sv->cfg.max_batch_systems)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "nrhs must be 1..max_batch_systems");
+    if (j0 < 0) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "j0 must be >= 0");
+    Matrix& m = it->second;
+    Request r;
+    r.mat = matrix;
+    r.n = m.n;
+    r.nrhs = nrhs;
+    tprh_status st = validate_n(sv, m.n, &r.s, &r.E, &r.L, &r.split);
+    if (st != TPRH_SUCCESS) return st;
+    const size_t xb = align_up(sizeof(float) * (size_t)nrhs * m.n);
+    r.bytes = 2 * xb + (r.split ? align_up(tprh::split_workspace_bytes(m.n / r.s, nrhs)) : 0);
+    if (r.bytes > sv->arena.cap())
+        return tprh::set_error(TPRH_ERROR_BUDGET, "the request needs more memory than the server's whole budget");
+    r.d_src = d;
+    r.x_dst = x;
+    r.rhs_seed = rhs_seed;
+    r.j0 = j0;
+    r.t_submit = sv->now();
+    const long long id = sv->next_req++;
+    sv->reqs[id] = r;
+    sv->queue.push_back(id);
+    m.refs++;
+    sv->st.submitted++;
+    *request = id;
+    lk.unlock();
+    sv->cv_work.notify_one();
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_request_info(tprh_server* sv, long long request, tprh_request_info* info) {
+    if (!sv || !info) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->reqs.find(request);
+    if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
+    const Request& r = it->second;
+    info->state = r.state;
+    info->n = r.n;
+    info->nrhs = r.nrhs;
+    info->t_submit = r.t_submit;
+    info->t_launch = r.t_launch;
+    info->t_done = r.t_done;
+    info->x = (r.state == kDone && r.holds_block) ? reinterpret_cast<const float*>(sv->base + r.x_off) : nullptr;
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_s) {
+    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    for (;;) {
+        auto it = sv->reqs.find(request);
+        if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
+        if (it->second.state == kDone) return TPRH_SUCCESS;
+        if (it->second.state == kFailed) return tprh::set_error(it->second.status, sv->worker_error);
+        if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+            auto jt = sv->reqs.find(request);
+            if (jt != sv->reqs.end() && jt->second.state == kDone) return TPRH_SUCCESS;
+            return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "timed out waiting for the request");
+        }
+    }
+}
+
+tprh_status tprh_server_poll(tprh_server* sv, long long* ids, int max, int* count) {
+    if (!sv || !count || (max > 0 && !ids)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    const int k = std::min<int>(max, (int)sv->completed.size());
+    for (int i = 0; i < k; i++) ids[i] = sv->completed[i];
+    sv->completed.erase(sv->completed.begin(), sv->completed.begin() + k);
+    *count = k;
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_copy_result(tprh_server* sv, long long request, float* dst) {
+    if (!sv || !dst) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    const void* src;
+    size_t bytes;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        auto it = sv->reqs.find(request);
+        if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
+        const Request& r = it->second;
+        if (r.state != kDone || !r.holds_block)
+            return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "the request holds no answers");
+        src = sv->base + r.x_off;
+        bytes = sizeof(float) * (size_t)r.n * r.nrhs;
+    }
+    // the block stays valid: only tprh_server_request_release frees it
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+    if (e != hipSuccess) return tprh::set_hip_error(e, "hipMemcpy(result)");
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_request_release(tprh_server* sv, long long request) {
+    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->reqs.find(request);
+    if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
+    Request& r = it->second;
+    if (r.state != kDone && r.state != kFailed)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "the request has not completed");
+    if (r.holds_block) {
+        sv->arena.release(r.off, r.bytes);
+        sv->cv_work.notify_one();  // queued requests may fit now
+    }
+    sv->reqs.erase(it);
+    return TPRH_SUCCESS;
+}
+
+double tprh_server_now(const tprh_server* sv) { return sv ? sv->now() : 0.0; }
+
+tprh_status tprh_server_stats_get(tprh_server* sv, tprh_server_stats* out) {
+    if (!sv || !out) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    *out = sv->st;
+    out->queued = (int)sv->queue.size();
+    out->bytes_in_use = sv->arena.used();
+    out->peak_bytes = sv->arena.peak();
+    out->budget = sv->arena.cap();
+    return TPRH_SUCCESS;
+}
+
+}  // extern "C"

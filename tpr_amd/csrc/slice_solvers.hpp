@@ -105,14 +105,16 @@ __device__ __forceinline__ void load_e(const _Float16* __restrict__ p, float (&v
 }
 
 // Coefficients stored as CT (float or _Float16), right-hand side as float.
+// coff: offset of the lane's coefficients, doff: of its right-hand side
+// (they differ when several right-hand sides share one matrix).
 template <int E, typename CT>
 __device__ __forceinline__ void load_raw_t(const CT* __restrict__ a, const CT* __restrict__ b,
-                                           const CT* __restrict__ c, const float* __restrict__ d, size_t off,
-                                           RawPart<E>& p) {
-    load_e<E>(a + off, p.a);
-    load_e<E>(c + off, p.c);
-    load_e<E>(d + off, p.d);
-    if (b != nullptr) load_e<E>(b + off, p.b);
+                                           const CT* __restrict__ c, const float* __restrict__ d, size_t coff,
+                                           size_t doff, RawPart<E>& p) {
+    load_e<E>(a + coff, p.a);
+    load_e<E>(c + coff, p.c);
+    load_e<E>(d + doff, p.d);
+    if (b != nullptr) load_e<E>(b + coff, p.b);
 }
 
 // Iterative refinement: replaces p.d by the residual r = d - A x of the

@@ -38,7 +38,7 @@ template <int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
 __global__ __launch_bounds__(256) void rows_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
                                                    const CT* __restrict__ c, const float* __restrict__ d,
                                                    const float* __restrict__ xin, float* __restrict__ rows, int n,
-                                                   int lg_m, long long nslices) {
+                                                   int lg_m, long long nslices, size_t cstride) {
     constexpr int S = E * L;
     const int l = threadIdx.x % L;
     const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;  // slice over the batch
@@ -46,8 +46,9 @@ __global__ __launch_bounds__(256) void rows_kernel(const CT* __restrict__ a, con
     const long long gs = valid ? g : 0;
     const size_t x0 = (size_t)(gs >> lg_m) * (size_t)n;
     const size_t off = (size_t)gs * S + (size_t)l * E;
+    const size_t coff = (size_t)(gs >> lg_m) * cstride + (off - x0);  // cstride 0: one shared matrix
     RawPart<E> raw;
-    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, coff, off, raw);
     float* row = rows + (size_t)gs * kRowWords;
     if constexpr (REFINE) {
         float xo[E], xl, xr;
@@ -130,7 +131,8 @@ template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFI
 __global__ __launch_bounds__(256) void stage3_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
                                                      const CT* __restrict__ c, const float* __restrict__ d,
                                                      const float* __restrict__ rows, const float* __restrict__ xed,
-                                                     float* __restrict__ x, int n, int lg_m, long long nslices) {
+                                                     float* __restrict__ x, int n, int lg_m, long long nslices,
+                                                     size_t cstride) {
     constexpr int S = E * L;
     const int l = threadIdx.x % L;
     const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;
@@ -139,8 +141,9 @@ __global__ __launch_bounds__(256) void stage3_kernel(const CT* __restrict__ a, c
     const int j = (int)(gs & ((1LL << lg_m) - 1));  // slice inside its system
     const size_t x0 = (size_t)(gs >> lg_m) * (size_t)n;
     const size_t off = (size_t)gs * S + (size_t)l * E;
+    const size_t coff = (size_t)(gs >> lg_m) * cstride + (off - x0);  // cstride 0: one shared matrix
     RawPart<E> raw;
-    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, raw);
+    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, coff, off, raw);
     float xo[E];
     if constexpr (REFINE) {
         // neighbours inside the slice belong to this wave, whose loads all
@@ -185,16 +188,18 @@ namespace {
 template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
 void launch_round(const SplitArgs& p, size_t base, int grid, int threads, int lg_m, long long nslices, int cs, int m,
                   int rthreads, size_t rshmem, hipStream_t stream) {
-    const CT* a = static_cast<const CT*>(p.a) + base;
-    const CT* c = static_cast<const CT*>(p.c) + base;
-    const CT* b = HAS_B ? static_cast<const CT*>(p.b) + base : nullptr;
+    const size_t cbase = p.shared_coef ? 0 : base;
+    const size_t cstride = p.shared_coef ? 0 : (size_t)p.n;
+    const CT* a = static_cast<const CT*>(p.a) + cbase;
+    const CT* c = static_cast<const CT*>(p.c) + cbase;
+    const CT* b = HAS_B ? static_cast<const CT*>(p.b) + cbase : nullptr;
     const float* d = p.d + base;
     float* x = p.x + base;
     hipLaunchKernelGGL((rows_kernel<E, L, MODE, HAS_B, CT, REFINE>), dim3(grid), dim3(threads), 0, stream, a, b, c, d,
-                       x, p.rows, p.n, lg_m, nslices);
+                       x, p.rows, p.n, lg_m, nslices, cstride);
     hipLaunchKernelGGL((reduced_kernel<METHOD>), dim3(cs), dim3(rthreads), rshmem, stream, p.rows, p.xed, m);
     hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE>), dim3(grid), dim3(threads), 0, stream,
-                       a, b, c, d, p.rows, p.xed, x, p.n, lg_m, nslices);
+                       a, b, c, d, p.rows, p.xed, x, p.n, lg_m, nslices, cstride);
 }
 
 template <int METHOD, int E, int L, int MODE>

@@ -13,6 +13,7 @@
 
 #include "launch_uneven.hpp"
 #include "slice_solvers.hpp"
+#include "solve_slot.hpp"
 
 namespace tprh {
 
@@ -21,26 +22,12 @@ __device__ __forceinline__ void run_task(const UnevenTask& tk, const int* __rest
                                          const long long* __restrict__ offsets, const float* __restrict__ a,
                                          const float* __restrict__ b, const float* __restrict__ c,
                                          const float* __restrict__ d, float* __restrict__ x, float* lds) {
-    const int m = tk.m;
-    const int n = m * E * L;
     const int slot = threadIdx.x >> tk.lg_lanes;
-    const int t = threadIdx.x & ((1 << tk.lg_lanes) - 1);
     const bool valid = slot < tk.count;
     const int sys = sys_list[tk.first + (valid ? slot : 0)];
     const size_t x0 = (size_t)offsets[sys];
-    const size_t off = x0 + (size_t)t * E;
-    RawPart<E> raw;
-    load_raw<E>(a, HAS_B ? b : nullptr, c, d, off, raw);
-    float A[E], C[E], R[E], X[E];
-    normalise<E, MODE>(raw, HAS_B, A, C, R);
-    float* my = lds + slot * lds_words_per_system(m);
-    if constexpr (METHOD == 0) {
-        ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
-    } else {
-        tpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
-    }
-    (void)n;
-    if (valid) store_e<E>(x + off, X);
+    solve_slot<METHOD, MODE, HAS_B, E, L>(tk.m, tk.lg_lanes, valid, a + x0, HAS_B ? b + x0 : nullptr, c + x0, d + x0,
+                                          x + x0, lds);
 }
 
 __global__ void zero_counter_kernel(int* counter) { *counter = 0; }
