@@ -25,6 +25,7 @@ per thread over the systems, on this host's cores, on a bounded sample of the
 same batch; the GPU answers of that sample are compared with it (rel-L2).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -49,6 +50,7 @@ def parse():
     ap.add_argument("--eager", action="store_true",
                     help="launch the timed steps one by one from Python instead of replaying a captured HIP graph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="config 3: systems per round of the split path")
@@ -120,7 +122,53 @@ def gpu_leg(args):
     torch.cuda.synchronize()
     x0 = sets[0]["x"].cpu().numpy()
     host = {k: sets[0][k].cpu().numpy() for k in ("a", "b", "c", "d", "xt")}
-    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host}
+    vendor = None if args.no_vendor else vendor_leg(sets, host)
+    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host, "vendor": vendor}
+
+
+def vendor_leg(sets, host):
+    """rocSPARSE's batched tridiagonal solver (no pivoting) on the same
+    buffer sets: the vendor comparator, 20 steps cycling through the sets,
+    launched eagerly (a library call is kept out of graph capture), timed
+    with HIP events, with its accuracy against the truth on set 0."""
+    import numpy as np
+    import torch
+
+    from tpr_amd import nstab
+    f = nstab._vendor()
+    if f is None:
+        return {"error": "libtpr_rocsparse.so not built"}
+
+    def step(i):
+        t = sets[i % ROT]
+        st = f(t["a"].data_ptr(), t["b"].data_ptr(), t["c"].data_ptr(), t["d"].data_ptr(), t["x"].data_ptr(), N,
+               NSYS, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if st != 0:
+            raise RuntimeError("rocSPARSE comparator failed (%d)" % st)
+
+    try:
+        reps = 20
+        for i in range(ROT):
+            step(i)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(reps):
+            step(i)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        step(0)
+        torch.cuda.synchronize()
+        x = sets[0]["x"].cpu().numpy().reshape(NSYS, N).astype(np.float64)
+        xt = host["xt"].reshape(NSYS, N).astype(np.float64)
+        err = np.linalg.norm(x - xt, axis=1) / np.linalg.norm(xt, axis=1)
+        return {"solver": "rocsparse_sgtsv_no_pivot_strided_batch", "ms_per_step": round(ms, 5),
+                "systems_per_s": round(NSYS / (ms / 1e3), 1),
+                "GBps_algorithmic": round(20 * N * NSYS / (ms / 1e3) / 1e9, 1),
+                "max_rel_l2_vs_truth": float(err.max())}
+    except Exception as e:  # the comparator is optional: report, do not fail the bench
+        return {"error": str(e)}
 
 
 def cpu_leg(args, host, x_gpu):
@@ -568,6 +616,8 @@ def main():
         "kernel_ms_avg": round(kernel_ms, 5),
         "wall_s": round(g["wall_s"], 4),
     }
+    if g.get("vendor") is not None:
+        out["vendor_comparator"] = g["vendor"]
     if not args.no_cpu:
         base, acc = cpu_leg(args, g["host"], g["x0"])
         out["cpu_baseline"] = base

@@ -133,12 +133,25 @@ def _status_name(st):
             4: "out of memory", 5: "budget exceeded"}.get(st, "status %d" % st)
 
 
+def _load_torch_first():
+    """PyTorch's wheel carries its own HIP runtime (soname libamdhip64.so.7,
+    requested by torch as libamdhip64.so).  Loaded after ours it would be a
+    second runtime in the process, which finds no GPU; loaded first, our
+    library's libamdhip64.so.7 dependency resolves to it.  So torch, when
+    installed, is imported before any library of this package is opened."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded library; raises RuntimeError if it was not built."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("tpr_amd/libtpr_hip.so is not built: run `python -m tpr_amd.build`")
+        _load_torch_first()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

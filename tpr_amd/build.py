@@ -27,6 +27,12 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtpr_hip.so")
 CLI_SRC = os.path.join(HERE, "cli", "tpr_pm.cpp")
 CLI_BIN = os.path.join(HERE, "tpr_pm")
+NSTAB_SRC = os.path.join(HERE, "cli", "tpr_nstab.cpp")
+NSTAB_BIN = os.path.join(HERE, "tpr_nstab")
+VENDOR_SRC = os.path.join(HERE, "comparators", "rocsparse_gtsv.cpp")
+VENDOR_LIB = os.path.join(HERE, "libtpr_rocsparse.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HOST_HIP = ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -73,6 +79,27 @@ def build_cli():
     return CLI_BIN
 
 
+def build_vendor():
+    """libtpr_rocsparse.so, the rocSPARSE comparator (host code only); skipped
+    when the image has no rocSPARSE."""
+    if not os.path.exists(os.path.join(ROCM, "include", "rocsparse", "rocsparse.h")):
+        return None
+    tmp = VENDOR_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC"] + HOST_HIP + [VENDOR_SRC, "-o", tmp,
+          "-L" + os.path.join(ROCM, "lib"), "-lrocsparse", "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, VENDOR_LIB)
+    return VENDOR_LIB
+
+
+def build_nstab():
+    tmp = NSTAB_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17"] + HOST_HIP + ["-I" + os.path.join(ROOT, "include"), NSTAB_SRC, "-o", tmp,
+          "-L" + HERE, "-ltpr_hip", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-ldl",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, NSTAB_BIN)
+    return NSTAB_BIN
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="library only")
@@ -80,9 +107,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
     print("built", build_lib(args.j))
     if not args.lib:
-        b = build_cli()
-        if b:
-            print("built", b)
+        for b in (build_cli(), build_vendor(), build_nstab()):
+            if b:
+                print("built", b)
 
 
 if __name__ == "__main__":
