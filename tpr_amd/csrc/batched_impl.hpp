@@ -19,7 +19,11 @@ namespace tprh {
 // to BLOCK lanes).  HAS_B: the diagonal is given and divided out in registers
 // (false: unit diagonal, b is not read).  CT: storage type of a, b, c (float
 // or _Float16).  REFINE: one iterative-refinement step, x += A^-1 (d - A x)
-// with the residual accumulated in FP64.
+// with the residual accumulated in FP64.  BLOCK == kSmallBlock: every system
+// fits in one wavefront (m * L <= 64) and its reduced solve synchronises
+// per wavefront, so the four wavefronts of a workgroup run independently.
+constexpr int kSmallBlock = 256;
+
 template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B, typename CT, bool REFINE>
 __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
                                                        const CT* __restrict__ c, const float* __restrict__ d,
@@ -45,10 +49,11 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
     }
     float A[E], C[E], R[E], X[E];
     normalise<E, MODE>(raw, HAS_B, A, C, R);
+    constexpr bool WAVE = BLOCK == kSmallBlock;
     if constexpr (METHOD == 0) {
-        ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
+        ptpr_solve_regs<E, L, MODE, WAVE>(A, C, R, m, t, my, X);
     } else {
-        tpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
+        tpr_solve_regs<E, L, MODE, WAVE>(A, C, R, m, t, my, X);
     }
     if constexpr (REFINE) {
 #pragma unroll
@@ -97,12 +102,13 @@ template <int METHOD, int E, int L, int MODE>
 hipError_t launch_el(const BatchArgs& p, hipStream_t stream) {
     const int T = p.m * L;  // lanes per system, a power of two
     const int lg = __builtin_ctz((unsigned)T);
-    const int block = T <= 64 ? 64 : T;
+    const int block = T <= 64 ? kSmallBlock : T;
     const int spb = block / T;
     const int grid = (p.nsys + spb - 1) / spb;
     const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(p.m);
-    hipError_t e = block == 64 ? launch_variant<METHOD, E, L, MODE, 64>(p, grid, block, shmem, lg, stream)
-                               : launch_variant<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
+    // the wavefront-synchronised variant only when a system fits one wavefront
+    hipError_t e = T <= 64 ? launch_variant<METHOD, E, L, MODE, kSmallBlock>(p, grid, block, shmem, lg, stream)
+                           : launch_variant<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }

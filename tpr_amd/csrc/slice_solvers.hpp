@@ -436,7 +436,7 @@ __device__ __forceinline__ void tpr_stage3_level(const float (&A)[E], const floa
 // system, stage 2.  On return xr[j] holds x at the end of slice j.
 // lds: lds_words_per_system(m) floats for this system.
 // (eA, eC, eR) is E_ed in slice lane L-1, (sA, sC, sR) is E_st in slice lane 0.
-template <int METHOD>  // 0 = PTPR (PCR stage 2), 1 = TPR (CR stage 2)
+template <int METHOD, bool WAVE = false>  // 0 = PTPR (PCR stage 2), 1 = TPR (CR stage 2)
 __device__ __forceinline__ const float* reduced_solve(float* lds, int m, int j, int l, int L, int t,
                                                       float eA, float eC, float eR, float sA, float sC,
                                                       float sR) {
@@ -459,7 +459,7 @@ __device__ __forceinline__ const float* reduced_solve(float* lds, int m, int j, 
         ed_c[j] = eC;
         ed_r[j] = eR;
     }
-    __syncthreads();
+    group_sync<WAVE>();
     if (l == 0) {
         // E_st += s1 * E_ed, src/ptpr.cpp:193-211 (src/tpr.cpp:327-340)
         float c_ed = ed_c[j];
@@ -468,7 +468,7 @@ __device__ __forceinline__ const float* reduced_solve(float* lds, int m, int j, 
         st_c[j] = s1;
         st_r[j] = sR + s1 * ed_r[j];
     }
-    __syncthreads();
+    group_sync<WAVE>();
     if (t < m) {
         // reduced row t from (E_ed(t), E_st(t+1)), src/ptpr.cpp:218-241
         if (t < m - 1) {
@@ -484,11 +484,11 @@ __device__ __forceinline__ const float* reduced_solve(float* lds, int m, int j, 
             rr0[t] = ed_r[t];
         }
     }
-    __syncthreads();
+    group_sync<WAVE>();
     if constexpr (METHOD == 0) {
-        return lds_pcr(ra0, rc0, rr0, ra1, rc1, rr1, m, t, m);
+        return lds_pcr<WAVE>(ra0, rc0, rr0, ra1, rc1, rr1, m, t, m);
     } else {
-        lds_cr(ra0, rc0, rr0, xo, m, t, m);
+        lds_cr<WAVE>(ra0, rc0, rr0, xo, m, t, m);
         return xo;
     }
 }
@@ -567,18 +567,19 @@ __device__ __forceinline__ void tpr_stage3(const float (&A)[E], const float (&C)
 // lane's E equations at element offset `off`.  sys_valid == false lanes run
 // the same instruction stream on zeros (every lane must take part in the
 // cross-lane exchanges and the barriers) and store nothing.
-template <int E, int L, int MODE>
+// WAVE: the system's lanes lie inside one wavefront (m * L <= 64).
+template <int E, int L, int MODE, bool WAVE = false>
 __device__ __forceinline__ void ptpr_solve_regs(float (&A)[E], float (&C)[E], float (&R)[E], int m, int t,
                                                 float* lds, float (&X)[E]) {
     const int l = t % L;
     const int j = t / L;
     ptpr_stage1<E, L, MODE>(A, C, R, l);
-    const float* xr = reduced_solve<0>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+    const float* xr = reduced_solve<0, WAVE>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
     const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
     ptpr_stage3<E, L, MODE>(A, C, R, xprev, xr[j], slice_end(lds, m, j), l, X);
 }
 
-template <int E, int L, int MODE>
+template <int E, int L, int MODE, bool WAVE = false>
 __device__ __forceinline__ void tpr_solve_regs(float (&A)[E], float (&C)[E], float (&R)[E], int m, int t,
                                                float* lds, float (&X)[E]) {
     const int l = t % L;
@@ -591,7 +592,7 @@ __device__ __forceinline__ void tpr_solve_regs(float (&A)[E], float (&C)[E], flo
         Ro[i] = R[i];
     }
     tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
-    const float* xr = reduced_solve<1>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+    const float* xr = reduced_solve<1, WAVE>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
     const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
     tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xr[j], l, X);
 }

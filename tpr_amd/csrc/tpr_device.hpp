@@ -188,13 +188,29 @@ __device__ __forceinline__ Eq pcr_update(Eq k, Eq kl, Eq kr, bool hasL, bool has
 }
 
 // ---------------------------------------------------------------------------
+// Synchronisation of the lanes that share one system's LDS rows.  WAVE: the
+// system lives inside one wavefront, whose LDS instructions execute in
+// program order, so a compiler fence at wavefront scope orders them and no
+// workgroup barrier is needed; otherwise a workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Reduced-system solvers on LDS (exact reference operation order).
 // Rows live in LDS arrays a[], c[], r[] of length m (a power of two); the
-// calling threads t = 0 .. nthr-1 of the workgroup cooperate; every thread of
-// the workgroup must call these (they contain __syncthreads()).
+// calling threads t = 0 .. nthr-1 cooperate; every thread of the group
+// (the wavefront when WAVE, else the workgroup) must call these.
 
 // PCR, src/pcr.cpp:15-89.  Double-buffered in (a, c, r) and (a1, c1, r1);
 // returns a pointer to the buffer that holds the answer in its r part.
+template <bool WAVE = false>
 __device__ __forceinline__ float* lds_pcr(float* a, float* c, float* r, float* a1, float* c1,
                                           float* r1, int m, int t, int nthr) {
     int pn = 31 - __builtin_clz(m);
@@ -227,7 +243,7 @@ __device__ __forceinline__ float* lds_pcr(float* a, float* c, float* r, float* a
             c1[k] = nc;
             r1[k] = nr;
         }
-        __syncthreads();
+        group_sync<WAVE>();
         float* t0 = a; a = a1; a1 = t0;
         t0 = c; c = c1; c1 = t0;
         t0 = r; r = r1; r1 = t0;
@@ -242,10 +258,11 @@ namespace tprh {
 // CR, src/cr.cpp:23-116, over m rows (a, c, r) in LDS; the answer goes to
 // xo[0..m).  Threads t = 0 .. nthr-1 cooperate; every thread of the
 // workgroup must call it.
+template <bool WAVE = false>
 __device__ __forceinline__ void lds_cr(float* a, float* c, float* r, float* xo, int m, int t, int nthr) {
     if (m <= 1) {
         if (t == 0) xo[0] = r[0];
-        __syncthreads();
+        group_sync<WAVE>();
         return;
     }
     const int lg = 31 - __builtin_clz(m);
@@ -276,13 +293,13 @@ __device__ __forceinline__ void lds_cr(float* a, float* c, float* r, float* xo, 
                     nr = inv * (r[k] - r[kl] * a[k]);
                 }
             }
-            __syncthreads();
+            group_sync<WAVE>();
             if (act) {
                 a[i] = na;
                 c[i] = nc;
                 r[i] = nr;
             }
-            __syncthreads();
+            group_sync<WAVE>();
         }
     }
     // backward substitution, src/cr.cpp:88-116
@@ -292,7 +309,7 @@ __device__ __forceinline__ void lds_cr(float* a, float* c, float* r, float* xo, 
         xo[i] = (r[i] - c[i] * r[i + u]) * inv_det;
         xo[i + u] = (r[i + u] - r[i] * a[i + u]) * inv_det;
     }
-    __syncthreads();
+    group_sync<WAVE>();
     for (int k = lg - 2; k >= 0; k--) {
         const int u = 1 << k;
         const int rows = m / (2 * u);
@@ -304,7 +321,7 @@ __device__ __forceinline__ void lds_cr(float* a, float* c, float* r, float* xo, 
                 xo[i] = r[i] - a[i] * xo[i - u] - c[i] * xo[i + u];
             }
         }
-        __syncthreads();
+        group_sync<WAVE>();
     }
 }
 
