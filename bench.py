@@ -123,7 +123,42 @@ def gpu_leg(args):
     x0 = sets[0]["x"].cpu().numpy()
     host = {k: sets[0][k].cpu().numpy() for k in ("a", "b", "c", "d", "xt")}
     vendor = None if args.no_vendor else vendor_leg(sets, host)
-    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host, "vendor": vendor}
+    other = None if args.eager else other_method_leg(args, sets, shape)
+    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host, "vendor": vendor, "other": other}
+
+
+def other_method_leg(args, sets, shape):
+    """The other method (TPR beside PTPR, or PTPR beside TPR) on the same
+    buffer sets with the same protocol (graph of K steps, HIP events): a
+    side figure, not the headline."""
+    import torch
+
+    import tpr_amd
+    method = "tpr" if args.method == "ptpr" else "ptpr"
+
+    def step(i):
+        t = sets[i % ROT]
+        tpr_amd.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=method, arith=args.arith,
+                              shape=shape)
+
+    for i in range(ROT):
+        step(i)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for i in range(args.steps):
+            step(i)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbps = 20.0 * N * NSYS / (ms / 1e3) / 1e9
+    return {"method": method, "ms_per_step": round(ms, 5), "systems_per_s": round(NSYS / (ms / 1e3), 1),
+            "GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBS, 4)}
 
 
 def vendor_leg(sets, host):
@@ -616,6 +651,8 @@ def main():
         "kernel_ms_avg": round(kernel_ms, 5),
         "wall_s": round(g["wall_s"], 4),
     }
+    if g.get("other") is not None:
+        out["other_method"] = g["other"]
     if g.get("vendor") is not None:
         out["vendor_comparator"] = g["vendor"]
     if not args.no_cpu:
