@@ -51,7 +51,8 @@ typedef enum tprh_status {
     TPRH_ERROR_NOT_SUPPORTED = 2, /* valid, but outside what this build solves */
     TPRH_ERROR_HIP = 3,           /* a HIP runtime call failed */
     TPRH_ERROR_OUT_OF_MEMORY = 4,
-    TPRH_ERROR_BUDGET = 5         /* a serving request exceeds the memory budget */
+    TPRH_ERROR_BUDGET = 5,        /* a serving request exceeds the memory budget */
+    TPRH_ERROR_TIMEOUT = 6        /* tprh_server_wait gave up before the request completed */
 } tprh_status;
 
 typedef enum tprh_method {
@@ -259,7 +260,7 @@ typedef struct tprh_request_info {
 
 tprh_status tprh_server_request_info(tprh_server *srv, long long request, tprh_request_info *info);
 /* Blocks until the request is done (TPRH_SUCCESS), failed (its error), or
- * timeout_s passes (TPRH_ERROR_NOT_SUPPORTED with a timeout message). */
+ * timeout_s passes (TPRH_ERROR_TIMEOUT; the request keeps going). */
 tprh_status tprh_server_wait(tprh_server *srv, long long request, double timeout_s);
 /* Ids of requests completed since the previous call (at most max). */
 tprh_status tprh_server_poll(tprh_server *srv, long long *ids, int max, int *count);

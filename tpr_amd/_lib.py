@@ -15,6 +15,7 @@ ERROR_NOT_SUPPORTED = 2
 ERROR_HIP = 3
 ERROR_OUT_OF_MEMORY = 4
 ERROR_BUDGET = 5
+ERROR_TIMEOUT = 6
 
 PTPR_M, TPR_M, PCR_M, CR_M = 0, 1, 2, 3
 ARITH_FAST, ARITH_EXACT = 0, 1
@@ -130,7 +131,7 @@ class TprError(RuntimeError):
 
 def _status_name(st):
     return {0: "success", 1: "invalid value", 2: "not supported", 3: "HIP error",
-            4: "out of memory", 5: "budget exceeded"}.get(st, "status %d" % st)
+            4: "out of memory", 5: "budget exceeded", 6: "timed out"}.get(st, "status %d" % st)
 
 
 def _load_torch_first():

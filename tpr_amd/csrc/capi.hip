@@ -335,6 +335,7 @@ const char* tprh_status_string(tprh_status status) {
         case TPRH_ERROR_HIP: return "HIP runtime error";
         case TPRH_ERROR_OUT_OF_MEMORY: return "out of memory";
         case TPRH_ERROR_BUDGET: return "memory budget exceeded";
+        case TPRH_ERROR_TIMEOUT: return "timed out";
     }
     return "unknown status";
 }

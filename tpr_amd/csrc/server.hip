@@ -710,7 +710,7 @@ tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_
         if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
             auto jt = sv->reqs.find(request);
             if (jt != sv->reqs.end() && jt->second.state == kDone) return TPRH_SUCCESS;
-            return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "timed out waiting for the request");
+            return tprh::set_error(TPRH_ERROR_TIMEOUT, "timed out waiting for the request");
         }
     }
 }
@@ -727,19 +727,16 @@ tprh_status tprh_server_poll(tprh_server* sv, long long* ids, int max, int* coun
 
 tprh_status tprh_server_copy_result(tprh_server* sv, long long request, float* dst) {
     if (!sv || !dst) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
-    const void* src;
-    size_t bytes;
-    {
-        std::lock_guard<std::mutex> g(sv->mu);
-        auto it = sv->reqs.find(request);
-        if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
-        const Request& r = it->second;
-        if (r.state != kDone || !r.holds_block)
-            return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "the request holds no answers");
-        src = sv->base + r.x_off;
-        bytes = sizeof(float) * (size_t)r.n * r.nrhs;
-    }
-    // the block stays valid: only tprh_server_request_release frees it
+    // under the lock, so that a concurrent tprh_server_request_release cannot
+    // free the block while it is copied
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->reqs.find(request);
+    if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
+    const Request& r = it->second;
+    if (r.state != kDone || !r.holds_block)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "the request holds no answers");
+    const void* src = sv->base + r.x_off;
+    const size_t bytes = sizeof(float) * (size_t)r.n * r.nrhs;
     hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
     if (e != hipSuccess) return tprh::set_hip_error(e, "hipMemcpy(result)");
     return TPRH_SUCCESS;
