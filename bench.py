@@ -474,78 +474,54 @@ def config2(args):
 
 
 def config4(args):
-    """The serving loop under load.  Requests arrive as a Poisson stream
-    (mean args.rate per second); each brings a matrix of N = 2^U{9..15}
-    equations (diagonally dominant generator, generated on the device and
-    resident in HBM) and 1..64 right-hand sides d ~ U(-1,1) (ADI-like
-    reuse of the matrix); the server admits them against a 64 GB budget and
-    batches continuously.  Reported: systems/s (right-hand sides solved per
-    second), p50/p99 latency from submit to completion, and, on the same
-    request mix, the saturated throughput (closed loop) and the plain
+    """The serving loop under load, driven by the C++ client
+    tpr_amd/tpr_serve_bench (one client thread; the Python client's own call
+    overhead would otherwise bound the saturated rate).  Requests arrive as a
+    Poisson stream (mean args.rate per second); each brings a matrix of
+    N = 2^U{9..15} equations (diagonally dominant generator, generated on the
+    device and resident in HBM) and 1..64 right-hand sides d ~ U(-1,1) (ADI-
+    like reuse of the matrix); the server admits them against a 64 GB budget
+    and batches continuously.  Reported: systems/s (right-hand sides solved
+    per second), p50/p99 latency from submit to completion; the same request
+    mix closed-loop with 256 outstanding (saturated throughput); the plain
     per-system GPU solve (one launch per right-hand side, the reference's
-    CUDA model)."""
+    CUDA model) on the first 300 requests; and the answers of a sample of
+    requests served through the Python binding against the oracle."""
+    import subprocess
+
     import numpy as np
+
+    exe = os.path.join(ROOT, "tpr_amd", "tpr_serve_bench")
+    if not os.path.exists(exe):
+        raise RuntimeError("tpr_amd/tpr_serve_bench is not built: run `python -m tpr_amd.build`")
+    common = ["--method", args.method, "--arith", args.arith, "--seed", "4044", "--budget-gb", "64"]
+
+    def client(extra):
+        p = subprocess.run([exe] + common + extra, capture_output=True, text=True, timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError("tpr_serve_bench failed: %s" % p.stderr[-2000:])
+        return json.loads(p.stdout.strip().splitlines()[-1])
+
+    # the clients run in child processes before this process touches the GPU
+    poisson = client(["--rate", str(args.rate), "--seconds", str(args.seconds)])
+    closed = client(["--rate", "0", "--requests", "20000", "--outstanding", "256"])
+
+    import ctypes
+
     import torch
 
+    from tpr_amd import _lib
     from tpr_amd import batched as B
     from tpr_amd.serve import Server
 
     torch.cuda.set_device(0)
     rng = np.random.default_rng(4044)
-    nreq = max(1, int(args.rate * args.seconds))
-    gaps = rng.exponential(1.0 / args.rate, nreq)
-    arrivals = np.cumsum(gaps)
-    sizes = (1 << rng.integers(9, 16, nreq)).astype(np.int64)
-    nrhs = rng.integers(1, 65, nreq).astype(np.int64)
-    check_every = max(1, nreq // 50)
-    srv = Server(64 << 30, method=args.method, arith=args.arith, max_inflight=4, max_batch_systems=16384)
-
-    def run(order, open_loop, max_outstanding=None):
-        """Serve requests `order`; open loop: at their arrival times."""
-        t_start = srv.now()
-        pending = {}
-        lat, checked = [], []
-        i = 0
-        done = 0
-        while done < len(order):
-            now = srv.now() - t_start
-            while i < len(order) and (not open_loop or arrivals[order[i]] - arrivals[order[0]] <= now) and \
-                    (max_outstanding is None or len(pending) < max_outstanding):
-                k = order[i]
-                m = srv.matrix_generate(int(sizes[k]), seed=4044, sys=k)
-                rid = srv.submit(m, int(nrhs[k]), rhs_seed=4044)
-                pending[rid] = (k, m)
-                i += 1
-            for rid in srv.poll():
-                k, m = pending.pop(rid)
-                inf = srv.info(rid)
-                lat.append(inf["t_done"] - inf["t_submit"])
-                if k % check_every == 0:
-                    checked.append((k, srv.result(rid)[:2].cpu().numpy()))
-                srv.release(rid)
-                srv.matrix_release(m)
-                done += 1
-        return srv.now() - t_start, np.array(lat), checked
-
-    # warm-up: every size class once
-    warm = [int(np.nonzero(sizes == (1 << e))[0][0]) for e in range(9, 16) if (sizes == (1 << e)).any()]
-    run(warm, False)
-    wall, lat, checked = run(list(range(nreq)), True)
-    systems = int(nrhs.sum())
-    eqs = int((sizes * nrhs).sum())
-    st = srv.stats()
-    # saturation: the same mix, closed loop with up to 256 requests outstanding
-    sat_n = min(nreq, 4000)
-    sat_wall, sat_lat, _ = run(list(range(sat_n)), False, max_outstanding=256)
-    sat_systems = int(nrhs[:sat_n].sum())
-    srv.close()
+    sizes = (1 << rng.integers(9, 16, 300)).astype(np.int64)
+    nrhs = rng.integers(1, 65, 300).astype(np.int64)
     # plain per-system solves: one launch per right-hand side (descriptors,
     # workspaces and inputs prepared beforehand; only the launches are timed)
-    import ctypes
-    from tpr_amd import _lib
-    base_n = min(nreq, 300)
     prepared = []
-    for k in range(base_n):
+    for k in range(len(sizes)):
         n = int(sizes[k])
         t = B.alloc_batch(1, n, with_xt=True)
         B.generate("dominant", 4044, k, 1, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
@@ -567,37 +543,42 @@ def config4(args):
         tot_sys += r
     torch.cuda.synchronize()
     base_wall = time.perf_counter() - t0
+    prepared = None
+
     out = {
-        "metric": METRIC, "value": round(systems / wall, 1), "unit": "systems/s", "n_gpus": 1, "steps": nreq,
-        "warmup": len(warm), "ms_per_step": round(1e3 * wall / nreq, 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "metric": METRIC, "value": poisson["systems_per_s"], "unit": "systems/s", "n_gpus": 1,
+        "steps": poisson["requests"], "warmup": 7,
+        "ms_per_step": round(1e3 * poisson["wall_s"] / poisson["requests"], 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "serving loop: Poisson arrivals at %.0f req/s for %.1f s (%d requests), N = 2^U{9..15}, "
                                "1..64 right-hand sides per matrix, matrices generated and resident in HBM, "
-                               "64 GB budget" % (args.rate, args.seconds, nreq),
+                               "64 GB budget, C++ client" % (args.rate, args.seconds, poisson["requests"]),
                    "method": args.method, "arith": args.arith, "max_inflight": 4},
-        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 4),
-                       "p99": round(1e3 * float(np.percentile(lat, 99)), 4),
-                       "max": round(1e3 * float(lat.max()), 4)},
-        "served": {"requests": nreq, "systems": systems, "equations": eqs, "wall_s": round(wall, 3),
-                   "batches": st["batches"], "max_batch_systems_seen": st["max_batch_systems_seen"],
-                   "max_inflight_seen": st["max_inflight_seen"], "peak_bytes": st["peak_bytes"]},
-        "saturated": {"requests": sat_n, "systems_per_s": round(sat_systems / sat_wall, 1),
-                      "p50_ms": round(1e3 * float(np.percentile(sat_lat, 50)), 4),
-                      "p99_ms": round(1e3 * float(np.percentile(sat_lat, 99)), 4),
-                      "GBps": round((12 * float((sizes[:sat_n]).sum()) + 8 * float((sizes[:sat_n] * nrhs[:sat_n]).sum()))
-                                    / sat_wall / 1e9, 1)},
-        "per_system_launches": {"requests": base_n, "systems_per_s": round(tot_sys / base_wall, 1)},
+        "latency_ms": poisson["latency_ms"],
+        "served": poisson,
+        "saturated": closed,
+        "per_system_launches": {"requests": len(sizes), "systems": tot_sys,
+                                "systems_per_s": round(tot_sys / base_wall, 1)},
     }
     if not args.no_cpu:
         import oracle as O
         worst = 0.0
-        for k, x in checked:
-            n = int(sizes[k])
-            a, b, c, _, _ = O.gen_system(4044, k, n)
-            for j in range(min(2, int(nrhs[k]))):
-                ref = O.solve_batch(args.method, a, b, c, O.gen_rhs(4044, k, j, n), n, min(n, 256))
-                worst = max(worst, O.rel_l2(x[j], ref))
-        out["accuracy"] = {"requests_checked": len(checked), "max_rel_l2_vs_oracle": worst, "tolerance": 1e-5}
+        with Server(8 << 30, method=args.method, arith=args.arith) as srv:
+            reqs = []
+            for k in range(0, len(sizes), 6):
+                m = srv.matrix_generate(int(sizes[k]), seed=4044, sys=k)
+                reqs.append((k, m, srv.submit(m, int(min(nrhs[k], 2)), rhs_seed=4044)))
+            for k, m, r in reqs:
+                srv.wait(r)
+                x = srv.result(r).cpu().numpy()
+                n = int(sizes[k])
+                a, b, c, _, _ = O.gen_system(4044, k, n)
+                for j in range(x.shape[0]):
+                    ref = O.solve_batch(args.method, a, b, c, O.gen_rhs(4044, k, j, n), n, min(n, 256))
+                    worst = max(worst, O.rel_l2(x[j], ref))
+                srv.release(r)
+                srv.matrix_release(m)
+        out["accuracy"] = {"requests_checked": len(reqs), "max_rel_l2_vs_oracle": worst, "tolerance": 1e-5}
     print(json.dumps(out))
 
 

@@ -29,6 +29,8 @@ CLI_SRC = os.path.join(HERE, "cli", "tpr_pm.cpp")
 CLI_BIN = os.path.join(HERE, "tpr_pm")
 NSTAB_SRC = os.path.join(HERE, "cli", "tpr_nstab.cpp")
 NSTAB_BIN = os.path.join(HERE, "tpr_nstab")
+SERVE_SRC = os.path.join(HERE, "cli", "tpr_serve_bench.cpp")
+SERVE_BIN = os.path.join(HERE, "tpr_serve_bench")
 VENDOR_SRC = os.path.join(HERE, "comparators", "rocsparse_gtsv.cpp")
 VENDOR_LIB = os.path.join(HERE, "libtpr_rocsparse.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -100,6 +102,14 @@ def build_nstab():
     return NSTAB_BIN
 
 
+def build_serve_bench():
+    tmp = SERVE_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17"] + HOST_HIP + ["-I" + os.path.join(ROOT, "include"), SERVE_SRC, "-o", tmp,
+          "-L" + HERE, "-ltpr_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, SERVE_BIN)
+    return SERVE_BIN
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="library only")
@@ -107,7 +117,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     print("built", build_lib(args.j))
     if not args.lib:
-        for b in (build_cli(), build_vendor(), build_nstab()):
+        for b in (build_cli(), build_vendor(), build_nstab(), build_serve_bench()):
             if b:
                 print("built", b)
 

@@ -1,0 +1,194 @@
+// tpr_serve_bench -- load generator for the serving loop (tprh_server_*).
+//
+//   tpr_serve_bench [--rate R] [--seconds T] [--requests K] [--outstanding Q]
+//                   [--min-log2 A] [--max-log2 B] [--max-rhs H] [--budget-gb G]
+//                   [--inflight F] [--method ptpr|tpr] [--arith fast|exact] [--seed S]
+//
+// Open loop (--rate R > 0): requests arrive as a Poisson stream of mean R per
+// second for T seconds.  Closed loop (--rate 0): K requests, at most Q
+// outstanding at any time (the server's saturated throughput).  Every request
+// brings a matrix of n = 2^U{A..B} equations, generated on the device from
+// (seed, request id) and resident in the server's arena, and 1..H right-hand
+// sides generated on the device (d ~ U(-1,1)); the answers stay in the arena
+// until the client sees the request complete, then request and matrix are
+// released.  Latency is submit to completion on the server's clock.
+// Prints one JSON line.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "tpr_hip.h"
+
+namespace {
+
+struct Args {
+    double rate = 2000.0, seconds = 5.0;
+    long long requests = 20000;
+    int outstanding = 256;
+    int min_log2 = 9, max_log2 = 15, max_rhs = 64;
+    double budget_gb = 64.0;
+    int inflight = 4;
+    tprh_method method = TPRH_PTPR;
+    tprh_arith arith = TPRH_ARITH_FAST;
+    unsigned long long seed = 4044;
+};
+
+[[noreturn]] void usage(const char* msg) {
+    std::fprintf(stderr, "tpr_serve_bench: %s\n", msg);
+    std::exit(2);
+}
+
+Args parse(int argc, char** argv) {
+    Args a;
+    for (int i = 1; i < argc; i++) {
+        std::string k = argv[i];
+        if (i + 1 >= argc) usage(("missing value for " + k).c_str());
+        const char* v = argv[++i];
+        if (k == "--rate") a.rate = std::atof(v);
+        else if (k == "--seconds") a.seconds = std::atof(v);
+        else if (k == "--requests") a.requests = std::atoll(v);
+        else if (k == "--outstanding") a.outstanding = std::atoi(v);
+        else if (k == "--min-log2") a.min_log2 = std::atoi(v);
+        else if (k == "--max-log2") a.max_log2 = std::atoi(v);
+        else if (k == "--max-rhs") a.max_rhs = std::atoi(v);
+        else if (k == "--budget-gb") a.budget_gb = std::atof(v);
+        else if (k == "--inflight") a.inflight = std::atoi(v);
+        else if (k == "--seed") a.seed = std::strtoull(v, nullptr, 10);
+        else if (k == "--method") {
+            if (!std::strcmp(v, "ptpr")) a.method = TPRH_PTPR;
+            else if (!std::strcmp(v, "tpr")) a.method = TPRH_TPR;
+            else usage("method must be ptpr or tpr");
+        } else if (k == "--arith") {
+            if (!std::strcmp(v, "fast")) a.arith = TPRH_ARITH_FAST;
+            else if (!std::strcmp(v, "exact")) a.arith = TPRH_ARITH_EXACT;
+            else usage("arith must be fast or exact");
+        } else usage(("unknown option " + k).c_str());
+    }
+    if (a.min_log2 < 4 || a.max_log2 < a.min_log2 || a.max_rhs < 1) usage("bad size or right-hand-side range");
+    return a;
+}
+
+void check(tprh_status st, const char* what) {
+    if (st != TPRH_SUCCESS) {
+        std::fprintf(stderr, "%s: %s (%s)\n", what, tprh_last_error(), tprh_status_string(st));
+        std::exit(1);
+    }
+}
+
+double pct(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = p / 100.0 * (double)(v.size() - 1);
+    const size_t lo = (size_t)pos;
+    const size_t hi = std::min(lo + 1, v.size() - 1);
+    return v[lo] + (pos - (double)lo) * (v[hi] - v[lo]);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const Args a = parse(argc, argv);
+    tprh_server_config cfg{};
+    cfg.method = a.method;
+    cfg.arith = a.arith;
+    cfg.hbm_budget = (size_t)(a.budget_gb * (double)(1ull << 30));
+    cfg.max_inflight = a.inflight;
+    cfg.max_batch_systems = 16384;
+    tprh_server* srv = nullptr;
+    check(tprh_server_create(&srv, &cfg), "tprh_server_create");
+
+    // the workload: arrival times (open loop), sizes and right-hand-side counts
+    std::mt19937_64 rng(a.seed);
+    std::uniform_int_distribution<int> lg(a.min_log2, a.max_log2), nr(1, a.max_rhs);
+    std::exponential_distribution<double> gap(a.rate > 0 ? a.rate : 1.0);
+    const bool open = a.rate > 0;
+    const long long nreq = open ? std::max(1LL, (long long)(a.rate * a.seconds)) : a.requests;
+    std::vector<double> arrive(nreq);
+    std::vector<int> n(nreq), rhs(nreq);
+    double t = 0.0;
+    long long systems = 0, equations = 0;
+    for (long long k = 0; k < nreq; k++) {
+        t += open ? gap(rng) : 0.0;
+        arrive[k] = t;
+        n[k] = 1 << lg(rng);
+        rhs[k] = nr(rng);
+        systems += rhs[k];
+        equations += (long long)rhs[k] * n[k];
+    }
+
+    // warm-up: one request of every size class, served alone
+    for (int e = a.min_log2; e <= a.max_log2; e++) {
+        long long m, r;
+        check(tprh_server_matrix_generate(srv, 1 << e, TPRH_GEN_DOMINANT, a.seed, 1ull << 40, &m), "matrix");
+        check(tprh_server_submit(srv, m, 1, nullptr, a.seed, 0, nullptr, &r), "submit");
+        check(tprh_server_wait(srv, r, 60.0), "wait");
+        check(tprh_server_request_release(srv, r), "release");
+        check(tprh_server_matrix_release(srv, m), "matrix release");
+    }
+    std::vector<long long> polled(4096);
+    int cnt = 0;
+    check(tprh_server_poll(srv, polled.data(), (int)polled.size(), &cnt), "poll");  // drop the warm-up ids
+
+    std::unordered_map<long long, long long> mat_of;  // request -> matrix
+    std::vector<double> lat;
+    lat.reserve(nreq);
+    double late_max = 0.0;
+    const double t0 = tprh_server_now(srv);
+    long long next = 0, done = 0;
+    while (done < nreq) {
+        const double now = tprh_server_now(srv) - t0;
+        while (next < nreq && (open ? arrive[next] <= now : (long long)mat_of.size() < a.outstanding)) {
+            if (open) late_max = std::max(late_max, now - arrive[next]);
+            long long m, r;
+            check(tprh_server_matrix_generate(srv, n[next], TPRH_GEN_DOMINANT, a.seed, (uint64_t)next, &m), "matrix");
+            check(tprh_server_submit(srv, m, rhs[next], nullptr, a.seed, 0, nullptr, &r), "submit");
+            mat_of[r] = m;
+            next++;
+        }
+        check(tprh_server_poll(srv, polled.data(), (int)polled.size(), &cnt), "poll");
+        for (int i = 0; i < cnt; i++) {
+            const long long r = polled[i];
+            tprh_request_info inf;
+            check(tprh_server_request_info(srv, r, &inf), "info");
+            if (inf.state != 2) {
+                std::fprintf(stderr, "request %lld failed\n", r);
+                return 1;
+            }
+            lat.push_back(inf.t_done - inf.t_submit);
+            check(tprh_server_request_release(srv, r), "release");
+            check(tprh_server_matrix_release(srv, mat_of[r]), "matrix release");
+            mat_of.erase(r);
+            done++;
+        }
+        if (cnt == 0) std::this_thread::yield();
+    }
+    const double wall = tprh_server_now(srv) - t0;
+    tprh_server_stats st;
+    check(tprh_server_stats_get(srv, &st), "stats");
+    check(tprh_server_destroy(srv), "destroy");
+    // HBM traffic the solves need: 12 B per matrix equation (a, b, c) and
+    // 8 B per right-hand-side equation (d read, x written)
+    double mat_eq = 0.0;
+    for (long long k = 0; k < nreq; k++) mat_eq += n[k];
+    const double bytes = 12.0 * mat_eq + 8.0 * (double)equations;
+    std::printf(
+        "{\"mode\": \"%s\", \"rate\": %.1f, \"requests\": %lld, \"systems\": %lld, \"equations\": %lld, "
+        "\"wall_s\": %.4f, \"requests_per_s\": %.1f, \"systems_per_s\": %.1f, \"GBps\": %.1f, "
+        "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"max\": %.4f}, \"max_arrival_lag_ms\": %.4f, "
+        "\"batches\": %lld, \"max_batch_systems_seen\": %lld, \"max_inflight_seen\": %d, \"peak_bytes\": %zu, "
+        "\"outstanding\": %d}\n",
+        open ? "poisson" : "closed", a.rate, nreq, systems, equations, wall, (double)nreq / wall,
+        (double)systems / wall, bytes / wall / 1e9, 1e3 * pct(lat, 50), 1e3 * pct(lat, 99),
+        1e3 * (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end())), 1e3 * late_max, st.batches,
+        st.max_batch_systems_seen, st.max_inflight_seen, st.peak_bytes, open ? 0 : a.outstanding);
+    return 0;
+}
