@@ -313,8 +313,20 @@ def config3(args):
                      "systems_per_s": round(nsys / (ms / 1e3), 1),
                      "GBps": round(20 * nsys * n / (ms / 1e3) / 1e9, 1), "x": None})
     host = {k: t[k].cpu().numpy() for k in ("a", "b", "c", "d")}
-    out = {"config": "config 3: %d weakly dominant fp32 systems x N=%d, S=%d (b = |a|+|c|+eps, eps ~ U[1e-4,1e-2])"
-                     % (nsys, n, s), "chunk": ch, "variants": variants}
+    head = variants[0]  # the fp32 solve is the line's value; the rest are variants of it
+    out = {
+        "metric": METRIC, "value": head["systems_per_s"], "unit": "systems/s", "n_gpus": 1, "steps": 10,
+        "warmup": 1, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d weakly dominant fp32 systems x N=%d, S=%d (b = |a|+|c|+eps, eps ~ U[1e-4,1e-2]), "
+                               "multi-workgroup path, one step = one batch solve" % (nsys, n, s),
+                   "method": "ptpr", "arith": "fast", "chunk": ch},
+        # algorithmic bytes 20 B/eq over the three kernels of a round (K1 rows,
+        # K2 reduced systems, K3 stage 3), per batch
+        "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "variants": variants,
+    }
     if not args.no_cpu:
         import oracle as O
         xf = O.thomas_f64_batch(host["a"], host["b"], host["c"], host["d"], n).reshape(nsys, n)
@@ -555,6 +567,10 @@ def config4(args):
                                "64 GB budget, C++ client" % (args.rate, args.seconds, poisson["requests"]),
                    "method": args.method, "arith": args.arith, "max_inflight": 4},
         "latency_ms": poisson["latency_ms"],
+        # latency-bound at the offered load; the solve traffic (12 B per matrix
+        # equation, 8 B per right-hand-side equation) against the HBM peak
+        "roofline": {"bound": "hbm", "achieved": poisson["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(poisson["GBps"] / HBM_PEAK_GBS, 6), "traffic": None},
         "served": poisson,
         "saturated": closed,
         "per_system_launches": {"requests": len(sizes), "systems": tot_sys,

@@ -106,8 +106,9 @@ class PCR : public SolverHandle {
 class CR : public SolverHandle {
    public:
     explicit CR(int n) : SolverHandle(TPRH_CR, n, 0) {}
-    // CR's setter ignores diag (src/cr.cpp:17-21); so does the C ABI for TPRH_CR
-    // src/cr.hpp:31-34 -> CR::set_tridiagonal_system ignores diag (src/cr.cpp:14-19)
+    // CR's 4-argument setter stores a, c, rhs and ignores diag
+    // (src/cr.cpp:17-21, reached from the constructor src/cr.hpp:31-34); so
+    // does the C ABI for TPRH_CR
     CR(float* a, float* diag, float* c, float* rhs, int n) : SolverHandle(TPRH_CR, n, 0) {
         (void)diag;
         set_tridiagonal_system(a, c, rhs);

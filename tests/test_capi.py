@@ -119,3 +119,32 @@ def test_workspace_sizing_without_device():
     d.refine, d.coef = 0, 0
     d.a = d.b = d.c = d.d = d.x = 16
     assert L.tprh_solve_ex(ctypes.byref(d), None) == _lib.ERROR_INVALID_VALUE
+
+
+def test_cpp_wrappers_compile_and_report_errors(tmp_path):
+    """include/tpr/hip_batch.hpp and hip_solvers.hpp compile against the C ABI
+    and turn a refused argument into tpr::hip::Error (no device needed)."""
+    import subprocess
+    src = tmp_path / "wrap.cpp"
+    src.write_text('''
+#include "tpr/hip_batch.hpp"
+#include "tpr/hip_solvers.hpp"
+int main() {
+    int bad = 0;
+    try { tpr::hip::UnevenPlan p({24}, TPRH_PTPR, TPRH_ARITH_FAST); }
+    catch (const tpr::hip::Error& e) { bad += e.status == TPRH_ERROR_INVALID_VALUE; }
+    try { tpr::hip::PTPR t(1000, 10); }
+    catch (const tpr::hip::Error& e) { bad += e.status == TPRH_ERROR_INVALID_VALUE; }
+    tprh_server_config cfg = {TPRH_PTPR, TPRH_ARITH_FAST, 16, 1, 1, nullptr};
+    try { tpr::hip::Server s(cfg); }
+    catch (const tpr::hip::Error& e) { bad += e.status == TPRH_ERROR_INVALID_VALUE; }
+    return bad == 3 ? 0 : 1;
+}
+''')
+    exe = tmp_path / "wrap"
+    lib = os.path.join(ROOT, "tpr_amd")
+    p = subprocess.run(["g++", "-std=c++17", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L" + lib, "-ltpr_hip", "-Wl,-rpath," + lib], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
