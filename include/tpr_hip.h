@@ -205,8 +205,9 @@ tprh_status tprh_generate(tprh_gen kind, uint64_t seed, uint64_t sys0, int nsys,
  * Matrices stay resident until released; a request solves nrhs right-hand
  * sides against one matrix.  A worker thread admits queued requests in FIFO
  * order as arena memory frees up, groups every admitted request into one
- * persistent launch (plus one split-path launch per request whose n exceeds
- * a workgroup), and keeps up to `max_inflight` batches on the device at once,
+ * persistent launch (plus one split-path launch set per size class whose n
+ * exceeds a workgroup; such classes need s >= 256), and keeps up to
+ * `max_inflight` batches on the device at once,
  * one stream each, so requests that arrive while others run are launched
  * beside them.  A system's answer depends only on its matrix, right-hand
  * side and size class (n -> s, E, L), never on the batch it rides in.

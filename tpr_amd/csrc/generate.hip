@@ -144,6 +144,37 @@ hipError_t launch_generate_coef(int kind, uint64_t seed, uint64_t sys, int n, fl
     return hipGetLastError();
 }
 
+// A flat grid over the concatenated values of every job: each thread finds
+// its job by binary search over the jobs' first indices.
+__global__ void generate_rhs_jobs_kernel(const RhsJob* __restrict__ jobs, int njobs, long long total,
+                                         float* __restrict__ base) {
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = njobs - 1;  // the last job with first <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].first <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const RhsJob& jb = jobs[lo];
+        const uint64_t e = (uint64_t)(g - jb.first);  // value index inside the job
+        const uint64_t key = mix64(jb.seed + kGolden * (jb.sys + 1));
+        const uint64_t jj = (uint64_t)jb.j0 + (e >> jb.lg_n);
+        const uint64_t idx = (e & ((1ULL << jb.lg_n) - 1));
+        base[jb.dst + (long long)e] = u_pm1(draw24(key, kStMulti, (jj << jb.lg_n) + idx));
+    }
+}
+
+hipError_t launch_generate_rhs_jobs(const RhsJob* jobs, int njobs, long long total, float* base,
+                                    hipStream_t stream) {
+    if (njobs == 0 || total == 0) return hipSuccess;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(generate_rhs_jobs_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, jobs, njobs, total,
+                       base);
+    return hipGetLastError();
+}
+
 hipError_t launch_generate(int kind, uint64_t seed, uint64_t sys0, int nsys, int n, float* a, float* b, float* c,
                            float* d, float* xt, hipStream_t stream) {
     const size_t total = (size_t)nsys * (size_t)n;

@@ -28,7 +28,33 @@ struct ServeLaunch {
     size_t lds_bytes;
 };
 
+// A system of a serving batch whose class needs the multi-workgroup path:
+// offsets in floats from the arena base; rows / xed are the system's part of
+// its request's workspace (m * 8 boundary floats, m slice-end values).
+struct ServeSplitSys {
+    long long coef;  // packed a | b | c, n floats each
+    long long rhs;
+    long long x;
+    long long rows;
+    long long xed;
+};
+
+struct ServeSplitLaunch {
+    const ServeSplitSys* sys;  // device table, nsys entries
+    int nsys;
+    int n;  // every system of one launch has the same size class
+    float* base;
+};
+
+// Slice shapes the serving batch's split launches carry (s >= 256, the
+// default shape of each s).
+inline bool serve_split_shape(int E, int L) {
+    return (E == 16 && (L == 16 || L == 32 || L == 64)) || (E == 32 && L == 64);
+}
+
 hipError_t launch_serve(int method, int mode, const ServeLaunch& p, hipStream_t stream);
+// The three split kernels over every system of the table (one size class).
+hipError_t launch_serve_split(int method, int mode, const ServeSplitLaunch& p, int E, int L, hipStream_t stream);
 hipError_t serve_occupancy(int method, int mode, size_t lds_bytes, int* blocks_per_cu);
 
 }  // namespace tprh

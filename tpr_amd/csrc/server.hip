@@ -9,10 +9,12 @@
 //      requests (answers copied out or retained, arena blocks freed);
 //   2. admission + launch: while a batch slot is free, queued requests are
 //      taken in FIFO order as long as the arena has room for them and the
-//      batch has room for their right-hand sides; the batch's work-item
-//      table is built on the host, copied through a pinned staging buffer,
-//      and one persistent serve kernel (plus a split-path launch for each
-//      request above one workgroup) runs on the slot's own stream;
+//      batch has room for their right-hand sides; the batch's tables are
+//      built on the host and copied through a pinned staging buffer in one
+//      transfer, and on the slot's own stream one launch generates the
+//      synthetic right-hand sides, one persistent serve kernel solves every
+//      class within a workgroup, and one split launch set per larger class
+//      solves the rest;
 //   3. idle: sleeps on a condition variable when nothing is queued or in
 //      flight, otherwise polls the in-flight events.
 //
@@ -132,8 +134,8 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    void* host_stage = nullptr;  // pinned: tasks | entries
-    void* dev_table = nullptr;   // device: tasks | entries | counter
+    void* host_stage = nullptr;  // pinned: the batch's packed tables
+    void* dev_table = nullptr;   // device: the same tables, then the work counter
     bool busy = false;
     std::vector<long long> reqs;
     long long systems = 0, equations = 0;
@@ -162,7 +164,7 @@ struct tprh_server {
     hipStream_t upload = nullptr;
     int grid = 1;
     size_t lds_bytes = 0;
-    size_t table_tasks_bytes = 0, table_bytes = 0;
+    size_t table_bytes = 0;  // capacity of a slot's packed tables; the counter follows
     std::thread worker;
     bool stop = false;
     std::string worker_error;
@@ -252,39 +254,70 @@ struct ClassKey {
 };
 
 // Builds and launches one batch on slot `sl` from the requests in `ids`
-// (already holding their arena blocks).  Returns the first HIP error.
+// (already holding their arena blocks).  Every table of the batch (work
+// items, serve-kernel entries, split entries, right-hand-side jobs) is
+// packed into the slot's pinned staging buffer and copied with one
+// transfer; then, on the slot's stream: caller right-hand sides are copied
+// in, synthetic ones generated (one launch), the serve kernel runs for every
+// class within a workgroup, one split launch set per larger class, and the
+// answers the caller asked for are copied out.  Returns the first HIP error.
 hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long long>& ids) {
     float* basef = reinterpret_cast<float*>(sv->base);
     const size_t fsz = sizeof(float);
-    // work items: group the right-hand sides of the non-split requests by class
-    std::map<ClassKey, std::vector<tprh::ServeSys>> groups;
-    std::map<int, std::pair<int, int>> shape_of;  // n -> (E, L)
     hipError_t e = hipSuccess;
+    std::map<ClassKey, std::vector<tprh::ServeSys>> groups;          // classes within a workgroup
+    std::map<int, std::pair<int, int>> shape_of;                      // n -> (E, L)
+    std::map<int, std::vector<tprh::ServeSplitSys>> split_groups;     // classes above a workgroup
+    std::vector<tprh::RhsJob> jobs;
+    long long rhs_total = 0;
     for (long long id : ids) {
         Request& r = sv->reqs[id];
         Matrix& m = sv->mats[r.mat];
         if (!m.ready_known) {
-            hipError_t q = hipEventQuery(m.ready);
-            if (q == hipSuccess) m.ready_known = true;
+            if (hipEventQuery(m.ready) == hipSuccess) m.ready_known = true;
             else if ((e = hipStreamWaitEvent(sl.stream, m.ready, 0)) != hipSuccess) return e;
         }
         const size_t nn = (size_t)r.n;
-        float* d = basef + r.off / fsz;
         if (r.d_src) {
-            e = hipMemcpyAsync(d, r.d_src, fsz * nn * r.nrhs, hipMemcpyDefault, sl.stream);
+            if ((e = hipMemcpyAsync(basef + r.off / fsz, r.d_src, fsz * nn * r.nrhs, hipMemcpyDefault, sl.stream)) !=
+                hipSuccess)
+                return e;
         } else {
-            e = tprh::launch_generate_rhs(r.rhs_seed, m.sys, r.j0, r.nrhs, r.n, d, sl.stream);
+            tprh::RhsJob jb{};
+            jb.seed = r.rhs_seed;
+            jb.sys = m.sys;
+            jb.dst = (long long)(r.off / fsz);
+            jb.first = rhs_total;
+            jb.j0 = r.j0;
+            jb.nrhs = r.nrhs;
+            jb.lg_n = __builtin_ctz((unsigned)r.n);
+            jobs.push_back(jb);
+            rhs_total += (long long)r.nrhs * r.n;
         }
-        if (e != hipSuccess) return e;
-        if (r.split) continue;
-        auto& g = groups[ClassKey{r.n}];
         shape_of[r.n] = {r.E, r.L};
-        for (int j = 0; j < r.nrhs; j++) {
-            tprh::ServeSys ss;
-            ss.coef = (long long)(m.off / fsz);
-            ss.rhs = (long long)(r.off / fsz + (size_t)j * nn);
-            ss.x = (long long)(r.x_off / fsz + (size_t)j * nn);
-            g.push_back(ss);
+        if (!r.split) {
+            auto& g = groups[ClassKey{r.n}];
+            for (int j = 0; j < r.nrhs; j++) {
+                tprh::ServeSys ss;
+                ss.coef = (long long)(m.off / fsz);
+                ss.rhs = (long long)(r.off / fsz + (size_t)j * nn);
+                ss.x = (long long)(r.x_off / fsz + (size_t)j * nn);
+                g.push_back(ss);
+            }
+        } else {
+            // the request's workspace: rows of its nrhs systems, then their x_ed
+            const size_t ms = (size_t)(r.n / r.s);
+            const size_t rows0 = r.ws_off / fsz;
+            auto& g = split_groups[r.n];
+            for (int j = 0; j < r.nrhs; j++) {
+                tprh::ServeSplitSys ss;
+                ss.coef = (long long)(m.off / fsz);
+                ss.rhs = (long long)(r.off / fsz + (size_t)j * nn);
+                ss.x = (long long)(r.x_off / fsz + (size_t)j * nn);
+                ss.rows = (long long)(rows0 + (size_t)j * ms * tprh::kSplitRowWords);
+                ss.xed = (long long)(rows0 + (size_t)r.nrhs * ms * tprh::kSplitRowWords + (size_t)j * ms);
+                g.push_back(ss);
+            }
         }
     }
     std::vector<tprh::UnevenTask> tasks;
@@ -307,46 +340,49 @@ hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long
             tasks.push_back(tk);
         }
     }
+    // pack the tables: each region's used part, 256-byte aligned
+    char* hs = static_cast<char*>(sl.host_stage);
+    char* dt = static_cast<char*>(sl.dev_table);
+    size_t at = 0;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t here = at;
+        if (bytes) std::memcpy(hs + here, src, bytes);
+        at = align_up(here + bytes);
+        return here;
+    };
+    const size_t o_tasks = put(tasks.data(), sizeof(tprh::UnevenTask) * tasks.size());
+    const size_t o_entries = put(entries.data(), sizeof(tprh::ServeSys) * entries.size());
+    std::vector<std::pair<size_t, size_t>> split_at;  // (offset, count) per split class, in map order
+    for (auto& kv : split_groups)
+        split_at.push_back({put(kv.second.data(), sizeof(tprh::ServeSplitSys) * kv.second.size()), kv.second.size()});
+    const size_t o_jobs = put(jobs.data(), sizeof(tprh::RhsJob) * jobs.size());
+    if (at > 0 && (e = hipMemcpyAsync(dt, hs, at, hipMemcpyHostToDevice, sl.stream)) != hipSuccess) return e;
+    if (!jobs.empty() &&
+        (e = tprh::launch_generate_rhs_jobs(reinterpret_cast<const tprh::RhsJob*>(dt + o_jobs), (int)jobs.size(),
+                                            rhs_total, basef, sl.stream)) != hipSuccess)
+        return e;
     if (!tasks.empty()) {
-        const size_t tb = sizeof(tprh::UnevenTask) * tasks.size();
-        const size_t eb = sizeof(tprh::ServeSys) * entries.size();
-        char* hs = static_cast<char*>(sl.host_stage);
-        std::memcpy(hs, tasks.data(), tb);
-        std::memcpy(hs + sv->table_tasks_bytes, entries.data(), eb);
-        char* dt = static_cast<char*>(sl.dev_table);
-        if ((e = hipMemcpyAsync(dt, hs, tb, hipMemcpyHostToDevice, sl.stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(dt + sv->table_tasks_bytes, hs + sv->table_tasks_bytes, eb, hipMemcpyHostToDevice,
-                                sl.stream)) != hipSuccess)
-            return e;
         tprh::ServeLaunch p;
-        p.tasks = reinterpret_cast<const tprh::UnevenTask*>(dt);
+        p.tasks = reinterpret_cast<const tprh::UnevenTask*>(dt + o_tasks);
         p.ntasks = (int)tasks.size();
         p.counter = reinterpret_cast<int*>(dt + sv->table_bytes);
-        p.sys = reinterpret_cast<const tprh::ServeSys*>(dt + sv->table_tasks_bytes);
+        p.sys = reinterpret_cast<const tprh::ServeSys*>(dt + o_entries);
         p.base = basef;
         p.grid = std::max(1, std::min(sv->grid, p.ntasks));
         p.lds_bytes = sv->lds_bytes;
         if ((e = tprh::launch_serve(sv->method, sv->mode, p, sl.stream)) != hipSuccess) return e;
     }
-    // requests above one workgroup: the split path, all right-hand sides on
-    // the one resident matrix
-    for (long long id : ids) {
-        Request& r = sv->reqs[id];
-        if (!r.split) continue;
-        Matrix& m = sv->mats[r.mat];
-        tprh::SplitArgs p{};
-        const float* a = basef + m.off / fsz;
-        p.a = a;
-        p.b = a + r.n;
-        p.c = a + 2 * (size_t)r.n;
-        p.d = basef + r.off / fsz;
-        p.x = basef + r.x_off / fsz;
-        p.n = r.n;
-        p.nsys = r.nrhs;
-        p.chunk = r.nrhs;
-        p.shared_coef = true;
-        tprh::split_carve(sv->base + r.ws_off, r.n / r.s, r.nrhs, &p);
-        if ((e = tprh::launch_split(sv->method, sv->mode, p, r.E, r.L, sl.stream)) != hipSuccess) return e;
+    size_t k = 0;
+    for (auto& kv : split_groups) {
+        tprh::ServeSplitLaunch p;
+        p.sys = reinterpret_cast<const tprh::ServeSplitSys*>(dt + split_at[k].first);
+        p.nsys = (int)split_at[k].second;
+        p.n = kv.first;
+        p.base = basef;
+        const auto sh = shape_of[kv.first];
+        if ((e = tprh::launch_serve_split(sv->method, sv->mode, p, sh.first, sh.second, sl.stream)) != hipSuccess)
+            return e;
+        k++;
     }
     for (long long id : ids) {
         Request& r = sv->reqs[id];
@@ -450,7 +486,13 @@ void worker_main(tprh_server* sv) {
 }
 
 tprh_status validate_n(tprh_server* sv, int n, int* s, int* E, int* L, bool* split) {
-    return tprh::class_shape(n, sv->have_schedule ? sv->schedule : nullptr, s, E, L, true, split);
+    tprh_status st = tprh::class_shape(n, sv->have_schedule ? sv->schedule : nullptr, s, E, L, true, split);
+    if (st != TPRH_SUCCESS) return st;
+    if (*split && !tprh::serve_split_shape(*E, *L))
+        return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "size classes above one workgroup need s >= 256 when served");
+    if (!*split && tprh::uneven_shape_id(*E, *L) < 0)
+        return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the serve kernel");
+    return TPRH_SUCCESS;
 }
 
 }  // namespace
@@ -479,6 +521,9 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
             if (st != TPRH_SUCCESS) return st;
             if (!split && tprh::uneven_shape_id(E, L) < 0)
                 return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the serve kernel");
+            if (split && !tprh::serve_split_shape(E, L))
+                return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED,
+                                       "size classes above one workgroup need s >= 256 when served");
         }
     }
     tprh_server* sv = new (std::nothrow) tprh_server();
@@ -511,9 +556,15 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
     sv->arena = Arena(cfg->hbm_budget);
     if ((e = hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking)) != hipSuccess)
         return bail(e, "hipStreamCreate");
-    // per-slot work-item tables: at most one item per right-hand side
-    sv->table_tasks_bytes = align_up(sizeof(tprh::UnevenTask) * (size_t)cfg->max_batch_systems);
-    sv->table_bytes = sv->table_tasks_bytes + align_up(sizeof(tprh::ServeSys) * (size_t)cfg->max_batch_systems);
+    // per-slot tables: work items (at most one per right-hand side), the
+    // serve kernel's system entries, the split launches' system entries
+    // (a batch has at most max_batch_systems right-hand sides, hence at most
+    // that many items, entries and requests; split classes add one alignment
+    // pad each, at most 32 of them)
+    sv->table_bytes = align_up(sizeof(tprh::UnevenTask) * (size_t)cfg->max_batch_systems) +
+                      align_up(sizeof(tprh::ServeSys) * (size_t)cfg->max_batch_systems) +
+                      align_up(sizeof(tprh::ServeSplitSys) * (size_t)cfg->max_batch_systems) + 32 * kAlign +
+                      align_up(sizeof(tprh::RhsJob) * (size_t)cfg->max_batch_systems);
     sv->slots.resize(cfg->max_inflight);
     for (Slot& sl : sv->slots) {
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
