@@ -169,15 +169,27 @@ tprh_status batched(tprh_method method, tprh_arith arith, const float* a, const 
     return solve_desc(desc, (hipStream_t)stream);
 }
 
-// Device buffers for the host-pointer entry points, grown on demand and kept
-// for the life of the process (the reference allocates and frees five
-// buffers per call, src/ptpr.cu:554-598).
-struct HostPathBuffers {
-    std::mutex mu;
+// Device buffers for the host-pointer entry points, one per device, grown on
+// demand and kept for the life of the process (the reference allocates and
+// frees five buffers per call, src/ptpr.cu:554-598).
+struct HostPathBuffer {
     float* dev = nullptr;
     size_t cap = 0;  // floats
 };
+struct HostPathBuffers {
+    std::mutex mu;
+    std::map<int, HostPathBuffer> per_device;
+};
 HostPathBuffers g_host_bufs;
+
+// Timing events of one host-path call, destroyed on every exit path.
+struct EventPair {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~EventPair() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
 
 tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a, const float* diag,
                             const float* c, const float* rhs, float* x, int n, int s) {
@@ -200,16 +212,19 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     } else {
         return fail(TPRH_ERROR_INVALID_VALUE, "unknown method");
     }
+    int device = 0;
+    TPRH_HIP(hipGetDevice(&device));
     std::lock_guard<std::mutex> lock(g_host_bufs.mu);
+    HostPathBuffer& buf = g_host_bufs.per_device[device];
     const size_t need = 5 * (size_t)n + (pl.ws + sizeof(float) - 1) / sizeof(float);
-    if (g_host_bufs.cap < need) {
-        if (g_host_bufs.dev) (void)hipFree(g_host_bufs.dev);
-        g_host_bufs.dev = nullptr;
-        g_host_bufs.cap = 0;
-        TPRH_HIP(hipMalloc(&g_host_bufs.dev, need * sizeof(float)));
-        g_host_bufs.cap = need;
+    if (buf.cap < need) {
+        if (buf.dev) (void)hipFree(buf.dev);
+        buf.dev = nullptr;
+        buf.cap = 0;
+        TPRH_HIP(hipMalloc(&buf.dev, need * sizeof(float)));
+        buf.cap = need;
     }
-    float* da = g_host_bufs.dev;
+    float* da = buf.dev;
     float* dc = da + n;
     float* dr = dc + n;
     float* dx = dr + n;
@@ -220,7 +235,8 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     TPRH_HIP(hipMemcpy(dc, c, bytes, hipMemcpyHostToDevice));
     TPRH_HIP(hipMemcpy(dr, rhs, bytes, hipMemcpyHostToDevice));
     if (diag) TPRH_HIP(hipMemcpy(db, diag, bytes, hipMemcpyHostToDevice));
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    EventPair timing;
+    hipEvent_t* ev = timing.ev;
     if (g_timing) {
         TPRH_HIP(hipEventCreate(&ev[0]));
         TPRH_HIP(hipEventCreate(&ev[1]));
@@ -247,8 +263,6 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
             float ms = -1.0f;
             if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) g_last_kernel_ms = ms;
         }
-        (void)hipEventDestroy(ev[0]);
-        (void)hipEventDestroy(ev[1]);
     }
     if (st != TPRH_SUCCESS) return st;
     TPRH_HIP(le);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 namespace tprh {
 
 // LDS words per system for the slice-boundary exchange and the reduced solve.
@@ -76,5 +78,19 @@ struct BatchArgs {
 hipError_t launch_batched(int method, int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
 hipError_t launch_batched_ptpr(int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
 hipError_t launch_batched_tpr(int mode, const BatchArgs& p, int E, int L, hipStream_t stream);
+
+// Raises a kernel's dynamic-LDS limit to `bytes`, once per device (a
+// function attribute is per device); thread-safe.  `done` is the caller's
+// per-kernel record of the devices already set.
+inline hipError_t ensure_lds_limit(const void* fn, int bytes, std::atomic<unsigned long long>& done) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const unsigned long long bit = dev < 64 ? 1ULL << dev : 0ULL;
+    if (bit && (done.load(std::memory_order_acquire) & bit)) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess && bit) done.fetch_or(bit, std::memory_order_acq_rel);
+    return e;
+}
 
 }  // namespace tprh

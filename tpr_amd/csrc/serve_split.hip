@@ -30,14 +30,10 @@ template <int METHOD, int E, int L, int MODE>
 hipError_t launch_el(const ServeSplitLaunch& p, hipStream_t stream) {
     const int m = p.n / (E * L);
     const int lg_m = __builtin_ctz((unsigned)m);
-    static bool attr_done = false;
-    if (!attr_done) {
-        const int bytes = (int)(sizeof(float) * 10 * kSplitMaxM);
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(reduced_kernel<METHOD, TableLoc>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e != hipSuccess) return e;
-        attr_done = true;
-    }
+    static std::atomic<unsigned long long> lds_set{0};  // K2 keeps 10 rows of m floats in LDS
+    hipError_t le = ensure_lds_limit(reinterpret_cast<const void*>(reduced_kernel<METHOD, TableLoc>),
+                                     (int)(sizeof(float) * 10 * kSplitMaxM), lds_set);
+    if (le != hipSuccess) return le;
     TableLoc loc{p.sys, p.base, p.n, m};
     const int threads = 256;
     const long long nslices = (long long)p.nsys * m;

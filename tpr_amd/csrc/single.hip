@@ -6,6 +6,7 @@
 // (src/cr.cpp:23-116).
 #include <hip/hip_runtime.h>
 
+#include "launch.hpp"
 #include "launch_gen.hpp"
 #include "tpr_device.hpp"
 
@@ -61,16 +62,12 @@ __global__ __launch_bounds__(1024) void cr_single_kernel(const float* __restrict
 
 hipError_t launch_single(int method, const float* a, const float* b, const float* c, const float* r, float* x, int n,
                          int nsys, hipStream_t stream) {
-    static bool attr_done = false;
-    if (!attr_done) {
-        // PCR keeps 6 rows of n floats in LDS: 96 KiB at n = 4096, above the
-        // 64 KiB default of dynamic LDS.
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(pcr_single_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(sizeof(float) * 6 * kSingleMaxN));
-        if (e != hipSuccess) return e;
-        attr_done = true;
-    }
+    // PCR keeps 6 rows of n floats in LDS: 96 KiB at n = 4096, above the
+    // 64 KiB default of dynamic LDS.
+    static std::atomic<unsigned long long> lds_set{0};
+    hipError_t le = ensure_lds_limit(reinterpret_cast<const void*>(pcr_single_kernel),
+                                     (int)(sizeof(float) * 6 * kSingleMaxN), lds_set);
+    if (le != hipSuccess) return le;
     int threads = n < 1024 ? n : 1024;
     threads = (threads + 63) / 64 * 64;
     if (method == 2) {

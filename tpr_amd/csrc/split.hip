@@ -62,14 +62,10 @@ hipError_t launch_split_el(const SplitArgs& p, hipStream_t stream) {
     const int lg_m = __builtin_ctz((unsigned)m);
     const int threads = 256;
     const int slices_per_block = threads / L;
-    static bool attr_done = false;
-    if (!attr_done) {
-        const int bytes = (int)(sizeof(float) * 10 * kSplitMaxM);
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(reduced_kernel<METHOD, BatchLoc<float>>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        if (e != hipSuccess) return e;
-        attr_done = true;
-    }
+    static std::atomic<unsigned long long> lds_set{0};  // K2 keeps 10 rows of m floats in LDS
+    hipError_t le = ensure_lds_limit(reinterpret_cast<const void*>(reduced_kernel<METHOD, BatchLoc<float>>),
+                                     (int)(sizeof(float) * 10 * kSplitMaxM), lds_set);
+    if (le != hipSuccess) return le;
     int rthreads = m < 1024 ? m : 1024;
     rthreads = (rthreads + 63) / 64 * 64;
     const size_t rshmem = sizeof(float) * 10 * (size_t)m;
