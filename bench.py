@@ -53,14 +53,15 @@ def parse():
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
-    ap.add_argument("--chunk", type=int, default=0, help="config 3: systems per round of the split path")
-    ap.add_argument("--config", type=int, default=1, choices=[1, 2, 3, 4],
-                    help="1: the headline batch (default); 2: 20000 systems of uneven N in one persistent launch; "
-                         "3: 256 x N=65536 weakly dominant, precision variants; 4: the serving loop under Poisson load")
-    ap.add_argument("--rate", type=float, default=2000.0, help="config 4: mean request arrivals per second")
-    ap.add_argument("--seconds", type=float, default=5.0, help="config 4: length of the Poisson run")
+    ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
+    ap.add_argument("--workload", default="batch", choices=["batch", "uneven", "precision", "serve"],
+                    help="batch: 4096 x N=1024, S=256 (default, the headline); uneven: 20000 systems of N = 2^U{8..14} "
+                         "in one persistent launch; precision: 256 weakly dominant x N=65536, fp16 storage and "
+                         "refinement variants; serve: the serving loop under Poisson load")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve workload: mean request arrivals per second")
+    ap.add_argument("--seconds", type=float, default=5.0, help="serve workload: length of the Poisson run")
     ap.add_argument("--retune", action="store_true",
-                    help="config 2: run the autotuner now instead of loading the committed schedule")
+                    help="uneven workload: run the autotuner now instead of loading the committed schedule")
     return ap.parse_args()
 
 
@@ -243,7 +244,7 @@ def cpu_leg(args, host, x_gpu):
     return base, acc
 
 
-def config3(args):
+def precision_workload(args):
     """256 weakly dominant systems x N=65536, S=256 (m = 256 slices: the
     multi-workgroup path).  Variants: fp32 PTPR, fp16-stored coefficients,
     each followed by 0..2 FP64-residual refinement steps against the fp32
@@ -352,12 +353,13 @@ def config3(args):
     print(json.dumps(out))
 
 
-def config2(args):
+def uneven_workload(args):
     """20000 systems, N = 2^U{8..14} (log-uniform over the powers of two),
     diagonally dominant generator, one persistent launch per step, replayed
     from a captured graph.  Schedules compared on the same batch:
     * autotuned: s and slice shape per size class from tpr_amd.autotune;
-    * config-1 schedule: s = min(N, 256) for every class, same kernel;
+    * uniform schedule: s = min(N, 256) for every class (the 4096 x N=1024
+      batch's slice size), same kernel;
     * grouped: the systems sorted by size and one batched launch per size
       class at s = min(N, 256) (the plain batched kernel, 7 launches).
     Algorithmic bytes: 20 B per equation, ~92.9 M equations per step."""
@@ -379,7 +381,7 @@ def config2(args):
         tuned_from = os.path.relpath(autotune.DEFAULT_PATH, ROOT)
     sched = autotune.schedule_of(table)
     plans = {"autotuned": UnevenPlan(sizes, args.method, args.arith, sched),
-             "config-1 schedule": UnevenPlan(sizes, args.method, args.arith, None)}
+             "uniform s=256 schedule": UnevenPlan(sizes, args.method, args.arith, None)}
     tot = plans["autotuned"].equations
     a, b, c, d, x, xt = (torch.empty(tot, device="cuda") for _ in range(6))
     plans["autotuned"].generate("dominant", 4242, 0, a, b, c, d, xt)
@@ -485,7 +487,7 @@ def config2(args):
     print(json.dumps(out))
 
 
-def config4(args):
+def serve_workload(args):
     """The serving loop under load, driven by the C++ client
     tpr_amd/tpr_serve_bench (one client thread; the Python client's own call
     overhead would otherwise bound the saturated rate).  Requests arrive as a
@@ -604,14 +606,14 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (systems are independent and "
               "would be sharded by count across GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.config == 3:
-        config3(args)
+    if args.workload == "precision":
+        precision_workload(args)
         return
-    if args.config == 2:
-        config2(args)
+    if args.workload == "uneven":
+        uneven_workload(args)
         return
-    if args.config == 4:
-        config4(args)
+    if args.workload == "serve":
+        serve_workload(args)
         return
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps
@@ -622,7 +624,7 @@ def main():
     kernel_ms = ms_per_step
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s_config1.json" % args.method)
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_batch4096x1024.json" % args.method)
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")

@@ -124,11 +124,11 @@ def test_alternative_shapes_bit_identical(s, E, L, method):
 
 
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
-def test_config1_full_size(method):
+def test_full_batch_4096x1024(method):
     """4096 systems x N = 1024, S = 256, generated on the GPU: exact is
     bit-identical to the oracle, fast within 1e-5 per system."""
     n, s, nsys = 1024, 256, 4096
-    say("config1", method)
+    say("4096 x 1024", method)
     t = B.alloc_batch(nsys, n)
     B.generate("dominant", 2024, 0, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
     a, b, c, d = (t[k].cpu().numpy() for k in "abcd")

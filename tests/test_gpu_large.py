@@ -137,7 +137,7 @@ def test_refinement_step_bit_identical(method, n, s, nsys):
 
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
 def test_refinement_reaches_fp64_accuracy(method):
-    """Weakly dominant systems (config 3 generator, smaller n): the fp16-stored
+    """Weakly dominant systems (the `weak` generator, smaller n): the fp16-stored
     solve is the least accurate, fp32 next; refinement steps against the fp32
     system bring both to the FP64 Thomas solution within fp32 rounding."""
     n, s, nsys = 16384, 256, 8
