@@ -69,3 +69,19 @@ def test_readme_invocation_config0():
 def test_invalid_shape_reports_error():
     p = run([1000, 256, 1, "cuptpr"])
     assert p.returncode != 0 and "[Error]" in p.stderr
+
+
+@pytest.mark.gpu
+def test_serve_bench_client_closed_loop():
+    """tpr_serve_bench, the C++ load generator of the serving loop: a short
+    closed-loop run completes every request and reports consistent counts."""
+    import json
+    exe = os.path.join(ROOT, "tpr_amd", "tpr_serve_bench")
+    p = subprocess.run([exe, "--rate", "0", "--requests", "300", "--outstanding", "32", "--budget-gb", "4",
+                        "--seed", "7"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    d = json.loads(p.stdout.strip().splitlines()[-1])
+    assert d["mode"] == "closed" and d["requests"] == 300
+    assert d["systems"] >= 300 and d["equations"] >= 300 * 512
+    assert d["systems_per_s"] > 0 and d["latency_ms"]["p50"] <= d["latency_ms"]["p99"] <= d["latency_ms"]["max"]
+    assert d["peak_bytes"] <= 4 << 30
