@@ -95,7 +95,7 @@ class TPR : public SolverHandle {
 class PCR : public SolverHandle {
    public:
     explicit PCR(int n) : SolverHandle(TPRH_PCR, n, 0) {}
-    // src/pcr.hpp:12-15 (the 5-argument constructor stores a, c, rhs; the
+    // src/pcr.hpp:14-17 (the 5-argument constructor stores a, c, rhs; the
     // division by diag is done by the 4-argument setter only)
     PCR(float* a, float* diag, float* c, float* rhs, int n) : SolverHandle(TPRH_PCR, n, 0) {
         (void)diag;

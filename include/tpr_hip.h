@@ -29,7 +29,7 @@
  *                                   a device-resident batch in one launch
  *   tprh_generate                   the seeded generator of the benchmark
  *                                   inputs (the reference's own generators,
- *                                   src/system.cpp:188-228, are unseeded)
+ *                                   src/system.cpp:118-158, are unseeded)
  *
  * All functions return a tprh_status; none of them exits the process (the
  * reference's CU_CHECK exits, src/ptpr.cu:509-517).  tprh_last_error() gives

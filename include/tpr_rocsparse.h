@@ -5,7 +5,7 @@
  *
  * Counterpart of the reference's cuSPARSE comparator
  * (src/reference_cusparse.cu:113-145, gtsv2_nopivot on one system, and its
- * nstab slot src/numerical_stability.cpp:180-182); rocSPARSE's strided
+ * nstab slot src/numerical_stability.cpp:191-193); rocSPARSE's strided
  * batch form solves a whole batch per call. */
 #ifndef TPR_ROCSPARSE_H
 #define TPR_ROCSPARSE_H

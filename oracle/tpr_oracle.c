@@ -509,7 +509,7 @@ static int tpr_run(tpr_ws *w, const float *a, const float *c, const float *rhs, 
     tpr_inter(w);
     tpr_stage2(w);
     tpr_stage3(w, omp);
-    memcpy(x, w->x, sizeof(real) * w->n); /* TPR::get_ans, src/tpr.cpp:400-406 */
+    memcpy(x, w->x, sizeof(real) * w->n); /* TPR::get_ans, src/tpr.cpp:396-401 */
     return orc_tpr_flops(w->n, w->s);
 }
 
@@ -677,8 +677,8 @@ enum { ST_A = 0, ST_C = 1, ST_B = 2, ST_XT = 3, ST_RHS = 4 };
 
 /* kind: 0 = strictly dominant (b = |a|+|c| + (1 + U[0,1)));
  *       1 = weakly dominant   (b = |a|+|c| + (1e-4 + U[0,1) * 0.0099));
- *       2 = ExampleFixedInput (a = c = -1/6, b = 1, rhs = i + 1; src/system.cpp:20-52... :188-200);
- *       3 = ExampleRandomRHSInput (a = c = -1/6, b = 1, rhs ~ U(-1,1); src/system.cpp:216-228).
+ *       2 = ExampleFixedInput (a = c = -1/6, b = 1, rhs = i + 1; src/system.cpp:118-130);
+ *       3 = ExampleRandomRHSInput (a = c = -1/6, b = 1, rhs ~ U(-1,1); src/system.cpp:146-158).
  * For kinds 0 and 1, xt ~ U(-1,1) and d = A*xt accumulated in double, rounded
  * to float.  For kinds 2 and 3, xt is left untouched (may be NULL). */
 void orc_gen_system(uint64_t seed, uint64_t sys, int n, int kind,

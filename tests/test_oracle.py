@@ -144,7 +144,7 @@ def test_known_answer_every_s(golden):
 
 
 def test_fixed_input_matches_reference_generator():
-    """ExampleFixedInput (src/system.cpp:188-200)."""
+    """ExampleFixedInput (src/system.cpp:118-130)."""
     a, b, c, d, _ = O.gen_system(0, 0, 16, O.GEN_FIXED)
     assert a[0] == 0 and c[-1] == 0
     assert np.all(a[1:] == np.float32(-1.0 / 6.0)) and np.all(c[:-1] == np.float32(-1.0 / 6.0))

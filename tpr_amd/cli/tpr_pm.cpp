@@ -13,7 +13,7 @@
 // solvers (tpr, pcr, ptpr, lapack) are not part of this GPU engine.
 //
 // Protocol of src/pm.cpp:226-239: per iteration the input is regenerated
-// (ExampleRandomRHSInput, src/system.cpp:216-228: a = c = -1/6, diag = 1,
+// (ExampleRandomRHSInput, src/system.cpp:146-158: a = c = -1/6, diag = 1,
 // rhs ~ U(-1,1)), outside the timed region, then solved from host arrays.
 // The time is the kernel only, from device events around the launch, as the
 // reference's DeviceTimer records it (src/ptpr.cu:574-587); one line
@@ -22,7 +22,7 @@
 //
 // Environment: TPR_ARITH=exact (default; bit-identical to the reference's CPU
 // solver) or fast; TPR_PM_INPUT=random (default) or fixed, the reference's
-// ExampleFixedInput (rhs = i + 1, src/system.cpp:188-200) whose answer for
+// ExampleFixedInput (rhs = i + 1, src/system.cpp:118-130) whose answer for
 // n = 1024 is test/ans1024.txt.
 #include <algorithm>
 #include <cctype>
@@ -55,7 +55,7 @@ bool parse_solver(std::string name, Solver* out) {
     return true;
 }
 
-struct Input {  // ExampleRandomRHSInput (src/system.cpp:216-228) / ExampleFixedInput
+struct Input {  // ExampleRandomRHSInput (src/system.cpp:146-158) / ExampleFixedInput
     Input(int n, bool fixed) : n(n), fixed(fixed), a(n), diag(n), c(n), rhs(n), rng(0), dist(-1.0, 1.0) {}
     void assign() {
         for (int i = 0; i < n; i++) {
