@@ -160,6 +160,8 @@ typedef struct tprh_solve_desc {
     int chunk;              /* systems per round of the multi-workgroup path, 0 = default */
     void *workspace;        /* device memory, needed when a system exceeds one workgroup */
     size_t workspace_bytes;
+    int coef_shared;        /* 1: every system uses the one matrix at a, b, c (n values each):
+                               nsys right-hand sides of one matrix, as in ADI sweeps */
 } tprh_solve_desc;
 
 /* Workspace bytes tprh_solve_ex needs for desc (0 when none); (size_t)-1 if

@@ -115,8 +115,12 @@ def test_workspace_sizing_without_device():
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
     d.s, d.refine, d.coef = 256, 1, _lib.STORE_F16                # refinement needs fp32 coefficients
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
+    d.refine, d.coef, d.coef_shared = 0, 0, 2                      # coef_shared is a flag
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
+    d.coef_shared = 1                                             # one matrix: same workspace per system
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 256
     # a large system without a workspace is refused before any launch
-    d.refine, d.coef = 0, 0
+    d.refine, d.coef, d.coef_shared = 0, 0, 0
     d.a = d.b = d.c = d.d = d.x = 16
     assert L.tprh_solve_ex(ctypes.byref(d), None) == _lib.ERROR_INVALID_VALUE
 

@@ -166,3 +166,53 @@ def test_refinement_reaches_fp64_accuracy(method):
     assert errs["f32+1"] <= 2e-7
     assert errs["f16+3"] <= 2e-7
     assert errs["f16"] > errs["f32"]
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nrhs,with_b", [(1024, 256, 37, True), (256, 16, 9, False), (65536, 256, 3, True),
+                                             (16384, 128, 4, True)])
+def test_shared_matrix_many_rhs_bit_identical(method, n, s, nrhs, with_b):
+    """coef_shared: nrhs right-hand sides against one matrix (ADI-style
+    reuse), through the one-workgroup and the multi-workgroup path; each
+    answer equals the oracle's solve of that right-hand side, bit for bit."""
+    say("shared matrix", method, n, s, nrhs, with_b)
+    a, b, c, _, _ = O.gen_system(36, 5, n, O.GEN_WEAK)
+    rhs = np.concatenate([O.gen_rhs(36, 5, j, n) for j in range(nrhs)])
+    if not with_b:
+        a, c, _ = O.normalize(a, b, c, np.zeros(n, np.float32))
+    x = torch.empty(nrhs * n, device="cuda")
+    B.solve_ex(dev(a), dev(b) if with_b else None, dev(c), dev(rhs), x, n, s, method=method, arith="exact",
+               shared=True)
+    torch.cuda.synchronize()
+    got = x.cpu().numpy().reshape(nrhs, n)
+    for j in range(nrhs):
+        r = rhs[j * n:(j + 1) * n]
+        if with_b:
+            ref = O.solve_batch(method, a, b, c, r, n, s)
+        else:
+            ref = (O.ptpr if method == "ptpr" else O.tpr)(a, c, r.copy(), s)
+        assert bits_equal(got[j], ref), j
+
+
+def test_shared_matrix_fp16_and_refinement():
+    n, s, nrhs = 2048, 256, 6
+    a, b, c, _, _ = O.gen_system(37, 2, n, O.GEN_WEAK)
+    rhs = np.concatenate([O.gen_rhs(37, 2, j, n) for j in range(nrhs)])
+    x = torch.empty(nrhs * n, device="cuda")
+    a16, b16, c16 = (v.astype(np.float16) for v in (a, b, c))
+    B.solve_ex(dev(a16, torch.float16), dev(b16, torch.float16), dev(c16, torch.float16), dev(rhs), x, n, s,
+               arith="exact", shared=True)
+    torch.cuda.synchronize()
+    got = x.cpu().numpy().reshape(nrhs, n)
+    wa, wb, wc = (v.astype(np.float32) for v in (a16, b16, c16))
+    for j in range(nrhs):
+        assert bits_equal(got[j], O.solve_batch("ptpr", wa, wb, wc, rhs[j * n:(j + 1) * n], n, s))
+    # one refinement step against the fp32 matrix, every right-hand side
+    x0 = x.cpu().numpy()
+    B.solve_ex(dev(a), dev(b), dev(c), dev(rhs), x, n, s, arith="exact", refine=True, shared=True)
+    torch.cuda.synchronize()
+    x1 = x.cpu().numpy().reshape(nrhs, n)
+    for j in range(nrhs):
+        sl = slice(j * n, (j + 1) * n)
+        ref = oracle_refine("ptpr", a, b, c, rhs[sl], x0[sl], n, s)
+        assert bits_equal(x1[j], ref), j

@@ -63,11 +63,15 @@ def _dptr_any(t, name, numel, dtypes):
     return t.data_ptr()
 
 
-def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0):
-    """tprh_solve_desc for torch tensors; a, b, c float32 or float16 (same dtype)."""
+def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
+              shared=False):
+    """tprh_solve_desc for torch tensors; a, b, c float32 or float16 (same
+    dtype).  shared: a, b, c hold one matrix (n values) for every right-hand
+    side in d."""
     import torch
     nsys = d.numel() // n
     tot = nsys * n
+    ctot = n if shared else tot
     coef16 = a.dtype == torch.float16
     cd = (torch.float16,) if coef16 else (torch.float32,)
     desc = _lib.SolveDesc()
@@ -75,14 +79,15 @@ def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sh
     desc.arith = ARITH[arith]
     desc.n, desc.s, desc.nsys = int(n), int(s), int(nsys)
     desc.coef = _lib.STORE_F16 if coef16 else _lib.STORE_F32
-    desc.a = _dptr_any(a, "a", tot, cd)
-    desc.b = _dptr_any(b, "b", tot, cd)
-    desc.c = _dptr_any(c, "c", tot, cd)
+    desc.a = _dptr_any(a, "a", ctot, cd)
+    desc.b = _dptr_any(b, "b", ctot, cd)
+    desc.c = _dptr_any(c, "c", ctot, cd)
     desc.d = _dptr_any(d, "d", tot, (torch.float32,))
     desc.x = _dptr_any(x, "x", tot, (torch.float32,))
     desc.refine = 1 if refine else 0
     desc.E, desc.L = shape if shape is not None else (0, 0)
     desc.chunk = int(chunk)
+    desc.coef_shared = 1 if shared else 0
     return desc
 
 
@@ -94,13 +99,14 @@ def workspace_bytes(desc):
 
 
 def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
-             workspace=None, stream=None):
+             workspace=None, stream=None, shared=False):
     """The general entry point (tprh_solve_ex): large systems (through a
     workspace, allocated here from torch if not given), fp16 coefficient
     storage (a, b, c as float16 tensors), one refinement step (refine=True:
-    x += A^-1 (d - A x) with an FP64 residual)."""
+    x += A^-1 (d - A x) with an FP64 residual), many right-hand sides of one
+    matrix (shared=True: a, b, c of n values, d and x of nsys * n)."""
     import torch
-    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk)
+    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared)
     need = workspace_bytes(desc)
     if need:
         if workspace is None:

@@ -28,7 +28,7 @@ template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B, typename CT
 __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
                                                        const CT* __restrict__ c, const float* __restrict__ d,
                                                        float* __restrict__ x, int n, int m, int nsys,
-                                                       int lg_lanes) {
+                                                       int lg_lanes, int shared) {
     extern __shared__ float lds[];
     const int sl = threadIdx.x >> lg_lanes;             // system slot in the workgroup
     const int t = threadIdx.x & ((1 << lg_lanes) - 1);  // lane inside the system
@@ -37,8 +37,9 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
     const bool valid = sys < nsys;
     const size_t x0 = valid ? (size_t)sys * (size_t)n : 0;
     const size_t off = x0 + (size_t)t * E;
+    const size_t coff = shared ? (size_t)t * E : off;  // shared: one matrix for every right-hand side
     RawPart<E> raw;
-    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, off, off, raw);
+    load_raw_t<E, CT>(a, HAS_B ? b : nullptr, c, d, coff, off, raw);
     float xo[E];
     if constexpr (REFINE) {
         // every lane of the system loads x here, before the barriers of the
@@ -71,10 +72,10 @@ void launch_b(const BatchArgs& p, int grid, int block, size_t shmem, int lg, hip
     const CT* c = static_cast<const CT*>(p.c);
     if (p.b != nullptr) {
         hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, true, CT, REFINE>), dim3(grid), dim3(block),
-                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg);
+                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg, p.shared_coef ? 1 : 0);
     } else {
         hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, false, CT, REFINE>), dim3(grid), dim3(block),
-                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg);
+                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg, p.shared_coef ? 1 : 0);
     }
 }
 

@@ -92,6 +92,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     if (d.arith != TPRH_ARITH_FAST && d.arith != TPRH_ARITH_EXACT) return fail(TPRH_ERROR_INVALID_VALUE, "bad arith");
     if (d.coef != TPRH_STORE_F32 && d.coef != TPRH_STORE_F16) return fail(TPRH_ERROR_INVALID_VALUE, "bad storage");
     if (d.refine != 0 && d.refine != 1) return fail(TPRH_ERROR_INVALID_VALUE, "refine must be 0 or 1");
+    if (d.coef_shared != 0 && d.coef_shared != 1) return fail(TPRH_ERROR_INVALID_VALUE, "coef_shared must be 0 or 1");
     if (d.refine && d.coef != TPRH_STORE_F32)
         return fail(TPRH_ERROR_INVALID_VALUE, "refinement needs fp32 coefficients for the residual");
     const bool explicit_shape = d.E != 0 || d.L != 0;
@@ -122,6 +123,7 @@ tprh_status solve_desc(const tprh_solve_desc& d, hipStream_t stream) {
         tprh::BatchArgs p{d.a, d.b, d.c, d.d, d.x, d.n, pl.m, d.nsys};
         p.f16 = d.coef == TPRH_STORE_F16;
         p.refine = d.refine != 0;
+        p.shared_coef = d.coef_shared != 0;
         e = tprh::launch_batched(method, mode, p, pl.E, pl.L, stream);
     } else {
         if (!d.workspace || d.workspace_bytes < pl.ws)
@@ -137,6 +139,7 @@ tprh_status solve_desc(const tprh_solve_desc& d, hipStream_t stream) {
         p.chunk = pl.chunk;
         p.f16 = d.coef == TPRH_STORE_F16;
         p.refine = d.refine != 0;
+        p.shared_coef = d.coef_shared != 0;
         tprh::split_carve(d.workspace, pl.m, pl.chunk, &p);
         e = tprh::launch_split(method, mode, p, pl.E, pl.L, stream);
     }

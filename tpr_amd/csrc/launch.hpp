@@ -72,6 +72,7 @@ struct BatchArgs {
     int nsys;  // systems
     bool f16 = false;     // a, b, c stored as fp16
     bool refine = false;  // x += A^-1 (d - A x), residual in FP64
+    bool shared_coef = false;  // every system uses the one matrix at a, b, c
 };
 
 // method 0 = PTPR, 1 = TPR; mode 0 = fast, 1 = exact.
