@@ -54,10 +54,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
-    ap.add_argument("--workload", default="batch", choices=["batch", "uneven", "precision", "serve"],
+    ap.add_argument("--workload", default="batch", choices=["batch", "uneven", "precision", "serve", "copy"],
                     help="batch: 4096 x N=1024, S=256 (default, the headline); uneven: 20000 systems of N = 2^U{8..14} "
                          "in one persistent launch; precision: 256 weakly dominant x N=65536, fp16 storage and "
-                         "refinement variants; serve: the serving loop under Poisson load")
+                         "refinement variants; serve: the serving loop under Poisson load; copy: a 256 MiB "
+                         "device copy, the practical HBM ceiling and the counters' calibration")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve workload: mean request arrivals per second")
     ap.add_argument("--seconds", type=float, default=5.0, help="serve workload: length of the Poisson run")
     ap.add_argument("--retune", action="store_true",
@@ -242,6 +243,37 @@ def cpu_leg(args, host, x_gpu):
     acc = {"rel_l2_vs_cpu_max": float(rel.max()), "rel_l2_vs_cpu_mean": float(rel.mean()),
            "bit_identical_to_cpu": bitexact, "rel_l2_vs_truth_max": float(err_t.max()), "cpu_check": kind}
     return base, acc
+
+
+COPY_BYTES = 256 << 20
+
+
+def copy_rate(steps=20):
+    """A plain 256 MiB device-to-device copy (torch's copy_), timed with HIP
+    events over `steps` copies: read + write bytes per second.  It is the
+    practical HBM ceiling the solver kernels are compared with, and the
+    kernel whose known bytes calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return 2.0 * COPY_BYTES / (ms / 1e3) / 1e9, ms
+
+
+def copy_workload(args):
+    torch_gbps, ms = copy_rate(args.steps)
+    print(json.dumps({"workload": "256 MiB device-to-device copy (torch copy_)", "bytes_read": COPY_BYTES,
+                      "bytes_written": COPY_BYTES, "ms_per_copy": round(ms, 5), "GBps": round(torch_gbps, 1),
+                      "frac_of_peak": round(torch_gbps / HBM_PEAK_GBS, 4)}))
 
 
 def precision_workload(args):
@@ -615,6 +647,9 @@ def main():
     if args.workload == "serve":
         serve_workload(args)
         return
+    if args.workload == "copy":
+        copy_workload(args)
+        return
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps
     value = NSYS / (ms_per_step / 1e3)
@@ -650,6 +685,9 @@ def main():
         "kernel_ms_avg": round(kernel_ms, 5),
         "wall_s": round(g["wall_s"], 4),
     }
+    cg, _ = copy_rate()
+    out["copy_ceiling"] = {"GBps": round(cg, 1), "frac": round(achieved / cg, 4),
+                           "what": "256 MiB device copy measured in this run (read + write bytes / time)"}
     if g.get("other") is not None:
         out["other_method"] = g["other"]
     if g.get("vendor") is not None:
