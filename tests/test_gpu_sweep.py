@@ -4,6 +4,7 @@ one-workgroup and the multi-workgroup path, both methods, with and without a
 diagonal.  Exact arithmetic must equal the oracle bit for bit on every
 system (strictly or weakly dominant inputs); fast arithmetic must stay
 within rel-L2 1e-5 of it on strictly dominant inputs."""
+import os
 import sys
 
 import numpy as np
@@ -51,7 +52,10 @@ def sweep_cases(count=48, seed=20251202):
     return cases
 
 
-@pytest.mark.parametrize("n,s,nsys,method,with_b,arith,kind", sweep_cases())
+# TPR_SWEEP_COUNT / TPR_SWEEP_SEED widen the sweep for a one-off run
+@pytest.mark.parametrize("n,s,nsys,method,with_b,arith,kind",
+                         sweep_cases(int(os.environ.get("TPR_SWEEP_COUNT", "48")),
+                                     int(os.environ.get("TPR_SWEEP_SEED", "20251202"))))
 def test_sweep(n, s, nsys, method, with_b, arith, kind):
     print("sweep", n, s, nsys, method, with_b, arith, kind, flush=True)
     sys.stdout.flush()

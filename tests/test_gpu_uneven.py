@@ -202,3 +202,49 @@ def test_autotune_table_runs_and_is_valid():
         assert tuple(cand) in autotune.candidates(n)
     plan = UnevenPlan([256, 2048, 256], schedule=sched)
     assert plan.equations == 256 + 2048 + 256
+
+
+def _random_plan(seed):
+    rng = np.random.default_rng(seed)
+    sizes = [1 << int(e) for e in rng.integers(4, 15, int(rng.integers(5, 60)))]
+    schedule = {}
+    for n in set(sizes):
+        cands = autotune.candidates(n)
+        schedule[n] = cands[int(rng.integers(0, len(cands)))]
+    return sizes, schedule, ["ptpr", "tpr"][int(rng.integers(0, 2))], bool(rng.integers(0, 2))
+
+
+# TPR_UNEVEN_SWEEP widens the sweep for a one-off run
+@pytest.mark.parametrize("seed", range(int(__import__("os").environ.get("TPR_UNEVEN_SWEEP", "6"))))
+def test_uneven_random_schedules_exact(seed):
+    """Random batches with a random menu shape per size class: every system
+    bit-identical to the oracle at its class's slice size."""
+    sizes, schedule, method, with_b = _random_plan(1000 + seed)
+    say("uneven sweep", seed, method, with_b, len(sizes), schedule)
+    plan = UnevenPlan(sizes, method=method, arith="exact", schedule=schedule)
+    a, b, c, d, x, xt = alloc(plan)
+    plan.generate("weak", 90 + seed, 0, a, b, c, d)
+    torch.cuda.synchronize()
+    h = [v.cpu().numpy() for v in (a, b, c, d)]
+    if not with_b:
+        na, nc, nd = [], [], []
+        for k in range(len(sizes)):
+            sa, sb, sc, sd = host_system(plan, k, h)
+            u, v, w = O.normalize(sa, sb, sc, sd)
+            na.append(u)
+            nc.append(v)
+            nd.append(w)
+        h = [np.concatenate(na), None, np.concatenate(nc), np.concatenate(nd)]
+        a, c, d = (torch.from_numpy(v).cuda() for v in (h[0], h[2], h[3]))
+        b = None
+    plan.solve(a, b, c, d, x)
+    torch.cuda.synchronize()
+    hx = x.cpu().numpy()
+    for k, n in enumerate(sizes):
+        lo, hi = plan.offset(k), plan.offset(k + 1)
+        s = schedule[n][0]
+        if with_b:
+            ref = O.solve_batch(method, h[0][lo:hi], h[1][lo:hi], h[2][lo:hi], h[3][lo:hi], n, s)
+        else:
+            ref = (O.ptpr if method == "ptpr" else O.tpr)(h[0][lo:hi], h[2][lo:hi], h[3][lo:hi].copy(), s)
+        assert bits_equal(hx[lo:hi], ref), (k, n, schedule[n])
