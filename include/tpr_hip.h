@@ -244,7 +244,8 @@ tprh_status tprh_server_matrix_release(tprh_server *srv, long long matrix);
 /* Queues nrhs right-hand sides against `matrix`.  d: nrhs * n floats (host
  * or device; copied when the request is launched, so it must stay valid
  * until then), or NULL to generate right-hand side j as
- * d ~ U(-1,1) of (rhs_seed, the matrix's generator id, j0 + j).  x: where
+ * d ~ U(-1,1) of (rhs_seed, the matrix's generator id, j0 + j) (id 0 for an
+ * uploaded matrix).  x: where
  * the answers go (nrhs * n floats, host or device) when the request
  * completes, or NULL to keep them in the arena until
  * tprh_server_request_release (tprh_server_request_info gives the device

@@ -176,3 +176,55 @@ def test_serve_errors():
         srv.matrix_release(m)
         with pytest.raises(_lib.TprError):
             srv.matrix_release(m)
+
+
+# TPR_SERVE_SWEEP widens the sweep for a one-off run
+@pytest.mark.parametrize("seed", range(int(__import__("os").environ.get("TPR_SERVE_SWEEP", "3"))))
+def test_serve_random_mix_exact(seed):
+    """Random request mixes under a tight budget (requests wait for memory),
+    random in-flight and batch limits, matrices uploaded or generated,
+    right-hand sides given or generated, answers retained or copied out:
+    every answer bit-identical to the oracle."""
+    rng = np.random.default_rng(500 + seed)
+    method = ["ptpr", "tpr"][int(rng.integers(0, 2))]
+    work = [(int(1 << rng.integers(4, 17)), int(rng.integers(1, 9))) for _ in range(int(rng.integers(10, 30)))]
+    need = max(2 * n * r * 4 + (n // 256 + 1) * r * 36 + 4096 for n, r in work)
+    # every matrix stays resident for the whole test (256-byte blocks); the
+    # requests share what is left, which holds only a few of them at a time
+    budget = int(need * rng.uniform(1.5, 4.0)) + sum(12 * n + 256 for n, _ in work)
+    say("serve sweep", seed, method, len(work), budget)
+    with Server(budget, method=method, arith="exact", max_inflight=int(rng.integers(1, 5)),
+                max_batch_systems=int(rng.integers(8, 64))) as srv:
+        # matrices first: creating one fails at once when the arena is full
+        # (requests wait, matrices do not), so none is created while
+        # requests hold memory
+        mats = []
+        for k, (n, nrhs) in enumerate(work):
+            a, b, c, _, _ = O.gen_system(SEED, 3000 + k, n)
+            uploaded = bool(rng.integers(0, 2))
+            if uploaded:
+                m = srv.matrix_upload(a, b, c)
+            else:
+                m = srv.matrix_generate(n, seed=SEED, sys=3000 + k)
+            mats.append((m, uploaded, (a, b, c)))
+        reqs = []
+        for k, (n, nrhs) in enumerate(work):
+            m, uploaded, (a, b, c) = mats[k]
+            given = bool(rng.integers(0, 2))
+            # generated right-hand sides use the matrix's generator id: 0 for an upload
+            gid = 3000 + k if (given or not uploaded) else 0
+            rhs = np.concatenate([O.gen_rhs(SEED + 1, gid, j, n) for j in range(nrhs)])
+            d = torch.from_numpy(rhs).cuda() if given else None
+            x = torch.empty(nrhs * n, device="cuda") if rng.integers(0, 2) else None
+            rid = srv.submit(m, nrhs, d=d, x=x, rhs_seed=SEED + 1)
+            reqs.append((k, n, nrhs, m, rid, x, (a, b, c), rhs))
+        for k, n, nrhs, m, rid, x, (a, b, c), rhs in reqs:
+            srv.wait(rid)
+            got = (x.cpu().numpy() if x is not None else srv.result(rid).cpu().numpy()).reshape(nrhs, n)
+            for j in range(nrhs):
+                ref = O.solve_batch(method, a, b, c, rhs[j * n:(j + 1) * n], n, min(n, 256))
+                assert bits_equal(got[j], ref), (k, n, j)
+            srv.release(rid)
+            srv.matrix_release(m)
+        st = srv.stats()
+        assert st["failed"] == 0 and st["peak_bytes"] <= budget and st["bytes_in_use"] == 0
