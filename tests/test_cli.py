@@ -85,3 +85,33 @@ def test_serve_bench_client_closed_loop():
     assert d["systems"] >= 300 and d["equations"] >= 300 * 512
     assert d["systems_per_s"] > 0 and d["latency_ms"]["p50"] <= d["latency_ms"]["p99"] <= d["latency_ms"]["max"]
     assert d["peak_bytes"] <= 4 << 30
+
+
+def _build_known_answer(tmp_path):
+    exe = tmp_path / "known_answer"
+    lib = os.path.join(ROOT, "tpr_amd")
+    p = subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "known_answer.cpp"), "-o", str(exe),
+                        "-L" + lib, "-ltpr_hip", "-Wl,-rpath," + lib], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    return exe
+
+
+def test_known_answer_program_builds(tmp_path):
+    """The C++ restatement of the reference's example tests compiles against
+    the headers and the library (its run is the gpu test below)."""
+    exe = _build_known_answer(tmp_path)
+    p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr
+
+
+@pytest.mark.gpu
+def test_known_answer_program(tmp_path):
+    """tests/cpp/known_answer.cpp: test/cuexamples.cpp and test/examples.cpp
+    in C++ on the engine, with the reference's own acceptance."""
+    exe = _build_known_answer(tmp_path)
+    p = subprocess.run([str(exe), os.path.join(ROOT, "tests", "golden", "ans1024.npy")], capture_output=True,
+                       text=True, timeout=120)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "0 failure(s)" in p.stdout
