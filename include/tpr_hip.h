@@ -223,6 +223,12 @@ typedef struct tprh_server_config {
     int max_inflight;       /* batches on the device at once (>= 1) */
     int max_batch_systems;  /* right-hand sides per batch (>= 1) */
     const int *schedule;    /* NULL or 32 (s, E, L) triples as tprh_uneven_plan_create */
+    /* Batching window.  With nothing on the device a batch is launched at
+     * once; while batches run, the next one waits until min_batch_systems
+     * right-hand sides are queued or the oldest queued request has waited
+     * max_wait_us (0, 0: launch whenever a slot is free). */
+    int min_batch_systems;
+    double max_wait_us;
 } tprh_server_config;
 
 tprh_status tprh_server_create(tprh_server **out, const tprh_server_config *cfg);
@@ -232,8 +238,10 @@ tprh_status tprh_server_destroy(tprh_server *srv);
 /* Resident matrices of n equations (n a power of two >= 16); fails with
  * TPRH_ERROR_BUDGET when the arena has no room.  _upload copies a, b, c
  * (host or device pointers; the copy is complete when the call returns);
- * _generate fills them on the device with the seeded generator (system id
- * `sys`, kinds as tprh_generate). */
+ * _generate makes no device call: the first batch that solves a request on
+ * the matrix fills it on the device with the seeded generator (system id
+ * `sys`, kinds as tprh_generate), in the same launch as the batch's
+ * synthetic right-hand sides. */
 tprh_status tprh_server_matrix_upload(tprh_server *srv, int n, const float *a, const float *b, const float *c,
                                       long long *matrix);
 tprh_status tprh_server_matrix_generate(tprh_server *srv, int n, tprh_gen kind, uint64_t seed, uint64_t sys,

@@ -24,6 +24,7 @@ pytestmark = pytest.mark.gpu
 
 FAST_TOL = 1e-5
 SEED = 515
+OKIND = {"dominant": O.GEN_DOMINANT, "weak": O.GEN_WEAK, "fixed": O.GEN_FIXED, "random_rhs": O.GEN_RANDOM_RHS}
 
 
 def say(*args):
@@ -156,6 +157,31 @@ def test_serve_budget_admission():
         for m in big + mats:
             srv.matrix_release(m)
         assert srv.stats()["bytes_in_use"] == 0
+
+
+@pytest.mark.parametrize("kind", ["dominant", "weak", "fixed", "random_rhs"])
+def test_serve_shared_generated_matrices(kind):
+    """Generated matrices are filled by the first batch that uses them; later
+    batches on other streams wait for that generation.  One request per
+    batch, several batches in flight, each matrix released right after its
+    last submit: every answer bit-identical to the oracle."""
+    n, nrhs, nmat, nreq = 2048, 3, 4, 32
+    say("serve shared generated", kind)
+    with Server(64 << 20, arith="exact", max_inflight=4, max_batch_systems=nrhs) as srv:
+        mats = [srv.matrix_generate(n, kind=kind, seed=SEED, sys=4000 + k) for k in range(nmat)]
+        rids = []
+        for q in range(nreq):
+            rids.append(srv.submit(mats[q % nmat], nrhs, rhs_seed=SEED, j0=q))
+            if q >= nreq - nmat:
+                srv.matrix_release(mats[q % nmat])  # freed when its last request retires
+        for q, rid in enumerate(rids):
+            srv.wait(rid)
+            ref = oracle_answers("ptpr", 4000 + q % nmat, n, nrhs, j0=q, kind=OKIND[kind])
+            assert bits_equal(srv.result(rid).cpu().numpy(), ref), q
+            srv.release(rid)
+        st = srv.stats()
+        assert st["batches"] == nreq and st["max_inflight_seen"] >= 2
+        assert st["failed"] == 0 and st["bytes_in_use"] == 0
 
 
 def test_serve_errors():

@@ -81,7 +81,8 @@ class ServerConfig(ctypes.Structure):
     """tprh_server_config of include/tpr_hip.h."""
     _fields_ = [("method", ctypes.c_int), ("arith", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("max_inflight", ctypes.c_int), ("max_batch_systems", ctypes.c_int),
-                ("schedule", ctypes.c_void_p)]
+                ("schedule", ctypes.c_void_p), ("min_batch_systems", ctypes.c_int),
+                ("max_wait_us", ctypes.c_double)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -1,8 +1,9 @@
 // tpr_serve_bench -- load generator for the serving loop (tprh_server_*).
 //
 //   tpr_serve_bench [--rate R] [--seconds T] [--requests K] [--outstanding Q]
-//                   [--min-log2 A] [--max-log2 B] [--max-rhs H] [--budget-gb G]
-//                   [--inflight F] [--method ptpr|tpr] [--arith fast|exact] [--seed S]
+//                   [--clients C] [--min-log2 A] [--max-log2 B] [--max-rhs H]
+//                   [--budget-gb G] [--inflight F] [--min-batch M] [--max-wait-us W]
+//                   [--method ptpr|tpr] [--arith fast|exact] [--seed S]
 //
 // Open loop (--rate R > 0): requests arrive as a Poisson stream of mean R per
 // second for T seconds.  Closed loop (--rate 0): K requests, at most Q
@@ -12,14 +13,19 @@
 // sides generated on the device (d ~ U(-1,1)); the answers stay in the arena
 // until the client sees the request complete, then request and matrix are
 // released.  Latency is submit to completion on the server's clock.
-// Prints one JSON line.
+// C client threads submit (request k belongs to client k mod C; in the closed
+// loop each keeps at most Q / C outstanding) and one collector thread takes
+// the completions, records latencies and releases; the server's calls are
+// thread-safe.  Prints one JSON line.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
@@ -34,9 +40,12 @@ struct Args {
     double rate = 2000.0, seconds = 5.0;
     long long requests = 20000;
     int outstanding = 256;
+    int clients = 1;
     int min_log2 = 9, max_log2 = 15, max_rhs = 64;
     double budget_gb = 64.0;
     int inflight = 4;
+    int min_batch = 0;
+    double max_wait_us = 0.0;
     tprh_method method = TPRH_PTPR;
     tprh_arith arith = TPRH_ARITH_FAST;
     unsigned long long seed = 4044;
@@ -57,11 +66,14 @@ Args parse(int argc, char** argv) {
         else if (k == "--seconds") a.seconds = std::atof(v);
         else if (k == "--requests") a.requests = std::atoll(v);
         else if (k == "--outstanding") a.outstanding = std::atoi(v);
+        else if (k == "--clients") a.clients = std::atoi(v);
         else if (k == "--min-log2") a.min_log2 = std::atoi(v);
         else if (k == "--max-log2") a.max_log2 = std::atoi(v);
         else if (k == "--max-rhs") a.max_rhs = std::atoi(v);
         else if (k == "--budget-gb") a.budget_gb = std::atof(v);
         else if (k == "--inflight") a.inflight = std::atoi(v);
+        else if (k == "--min-batch") a.min_batch = std::atoi(v);
+        else if (k == "--max-wait-us") a.max_wait_us = std::atof(v);
         else if (k == "--seed") a.seed = std::strtoull(v, nullptr, 10);
         else if (k == "--method") {
             if (!std::strcmp(v, "ptpr")) a.method = TPRH_PTPR;
@@ -74,6 +86,8 @@ Args parse(int argc, char** argv) {
         } else usage(("unknown option " + k).c_str());
     }
     if (a.min_log2 < 4 || a.max_log2 < a.min_log2 || a.max_rhs < 1) usage("bad size or right-hand-side range");
+    if (a.clients < 1 || a.clients > 64) usage("clients must be 1..64");
+    if (a.outstanding < a.clients) usage("outstanding must be at least the number of clients");
     return a;
 }
 
@@ -103,6 +117,8 @@ int main(int argc, char** argv) {
     cfg.hbm_budget = (size_t)(a.budget_gb * (double)(1ull << 30));
     cfg.max_inflight = a.inflight;
     cfg.max_batch_systems = 16384;
+    cfg.min_batch_systems = a.min_batch;
+    cfg.max_wait_us = a.max_wait_us;
     tprh_server* srv = nullptr;
     check(tprh_server_create(&srv, &cfg), "tprh_server_create");
 
@@ -138,39 +154,82 @@ int main(int argc, char** argv) {
     int cnt = 0;
     check(tprh_server_poll(srv, polled.data(), (int)polled.size(), &cnt), "poll");  // drop the warm-up ids
 
-    std::unordered_map<long long, long long> mat_of;  // request -> matrix
+    struct Owner {
+        long long matrix;
+        int client;
+    };
+    std::mutex mu;                                // guards owner
+    std::unordered_map<long long, Owner> owner;   // request -> (matrix, client)
+    std::vector<std::atomic<int>> outstanding(a.clients);
+    for (auto& o : outstanding) o.store(0);
+    const int per_client = a.outstanding / a.clients;
     std::vector<double> lat;
     lat.reserve(nreq);
-    double late_max = 0.0;
+    std::atomic<long long> late_ns{0};
+    std::atomic<bool> failed{false};
     const double t0 = tprh_server_now(srv);
-    long long next = 0, done = 0;
-    while (done < nreq) {
-        const double now = tprh_server_now(srv) - t0;
-        while (next < nreq && (open ? arrive[next] <= now : (long long)mat_of.size() < a.outstanding)) {
-            if (open) late_max = std::max(late_max, now - arrive[next]);
-            long long m, r;
-            check(tprh_server_matrix_generate(srv, n[next], TPRH_GEN_DOMINANT, a.seed, (uint64_t)next, &m), "matrix");
-            check(tprh_server_submit(srv, m, rhs[next], nullptr, a.seed, 0, nullptr, &r), "submit");
-            mat_of[r] = m;
-            next++;
-        }
-        check(tprh_server_poll(srv, polled.data(), (int)polled.size(), &cnt), "poll");
-        for (int i = 0; i < cnt; i++) {
-            const long long r = polled[i];
-            tprh_request_info inf;
-            check(tprh_server_request_info(srv, r, &inf), "info");
-            if (inf.state != 2) {
-                std::fprintf(stderr, "request %lld failed\n", r);
-                return 1;
+
+    auto client = [&](int c) {
+        for (long long k = c; k < nreq && !failed.load(); k += a.clients) {
+            if (open) {
+                while (tprh_server_now(srv) - t0 < arrive[k]) std::this_thread::yield();
+                const long long lag = (long long)(1e9 * (tprh_server_now(srv) - t0 - arrive[k]));
+                long long prev = late_ns.load();
+                while (lag > prev && !late_ns.compare_exchange_weak(prev, lag)) {
+                }
+            } else {
+                while (outstanding[c].load() >= per_client && !failed.load()) std::this_thread::yield();
             }
-            lat.push_back(inf.t_done - inf.t_submit);
-            check(tprh_server_request_release(srv, r), "release");
-            check(tprh_server_matrix_release(srv, mat_of[r]), "matrix release");
-            mat_of.erase(r);
-            done++;
+            long long m, r;
+            check(tprh_server_matrix_generate(srv, n[k], TPRH_GEN_DOMINANT, a.seed, (uint64_t)k, &m), "matrix");
+            {
+                // the collector looks a completed id up under the same lock,
+                // so it never sees a request before its owner is recorded
+                std::lock_guard<std::mutex> g(mu);
+                check(tprh_server_submit(srv, m, rhs[k], nullptr, a.seed, 0, nullptr, &r), "submit");
+                owner[r] = Owner{m, c};
+                outstanding[c]++;
+            }
         }
-        if (cnt == 0) std::this_thread::yield();
-    }
+    };
+    auto collector = [&]() {
+        std::vector<long long> ids(4096);
+        long long done = 0;
+        while (done < nreq && !failed.load()) {
+            int got = 0;
+            check(tprh_server_poll(srv, ids.data(), (int)ids.size(), &got), "poll");
+            for (int i = 0; i < got; i++) {
+                const long long r = ids[i];
+                Owner o;
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    auto it = owner.find(r);
+                    o = it->second;
+                    owner.erase(it);
+                }
+                tprh_request_info inf;
+                check(tprh_server_request_info(srv, r, &inf), "info");
+                if (inf.state != 2) {
+                    std::fprintf(stderr, "request %lld failed\n", r);
+                    failed.store(true);
+                    return;
+                }
+                lat.push_back(inf.t_done - inf.t_submit);
+                check(tprh_server_request_release(srv, r), "release");
+                check(tprh_server_matrix_release(srv, o.matrix), "matrix release");
+                outstanding[o.client]--;
+                done++;
+            }
+            if (got == 0) std::this_thread::yield();
+        }
+    };
+    std::thread coll(collector);
+    std::vector<std::thread> cl;
+    for (int c = 0; c < a.clients; c++) cl.emplace_back(client, c);
+    for (auto& th : cl) th.join();
+    coll.join();
+    if (failed.load()) return 1;
+    const double late_max = 1e-9 * (double)late_ns.load();
     const double wall = tprh_server_now(srv) - t0;
     tprh_server_stats st;
     check(tprh_server_stats_get(srv, &st), "stats");
@@ -185,10 +244,11 @@ int main(int argc, char** argv) {
         "\"wall_s\": %.4f, \"requests_per_s\": %.1f, \"systems_per_s\": %.1f, \"GBps\": %.1f, "
         "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"max\": %.4f}, \"max_arrival_lag_ms\": %.4f, "
         "\"batches\": %lld, \"max_batch_systems_seen\": %lld, \"max_inflight_seen\": %d, \"peak_bytes\": %zu, "
-        "\"outstanding\": %d}\n",
+        "\"outstanding\": %d, \"clients\": %d, \"min_batch\": %d, \"max_wait_us\": %.1f}\n",
         open ? "poisson" : "closed", a.rate, nreq, systems, equations, wall, (double)nreq / wall,
         (double)systems / wall, bytes / wall / 1e9, 1e3 * pct(lat, 50), 1e3 * pct(lat, 99),
         1e3 * (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end())), 1e3 * late_max, st.batches,
-        st.max_batch_systems_seen, st.max_inflight_seen, st.peak_bytes, open ? 0 : a.outstanding);
+        st.max_batch_systems_seen, st.max_inflight_seen, st.peak_bytes, open ? 0 : a.outstanding, a.clients,
+        a.min_batch, a.max_wait_us);
     return 0;
 }

@@ -117,37 +117,26 @@ __global__ void generate_rhs_kernel(uint64_t seed, uint64_t sys, int j0, int nrh
     }
 }
 
-// The coefficients of system `sys` only, packed [a | b | c] (n floats each):
-// the same values generate_one writes.
-__global__ void generate_coef_kernel(int kind, uint64_t seed, uint64_t sys, int n, float* __restrict__ abc) {
-    const uint64_t key = mix64(seed + kGolden * (sys + 1));
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float ai, bi, ci;
-        if (kind == 2 || kind == 3) {
-            ai = i == 0 ? 0.0f : (float)(-1.0 / 6.0);
-            ci = i == n - 1 ? 0.0f : (float)(-1.0 / 6.0);
-            bi = 1.0f;
-        } else {
-            gen_coef(key, i, n, kind, ai, bi, ci);
-        }
-        abc[i] = ai;
-        abc[n + i] = bi;
-        abc[2 * (size_t)n + i] = ci;
+// Equation i of the coefficients of system `sys` only, packed [a | b | c]
+// (n floats each): the same values generate_one writes.
+__device__ __forceinline__ void coef_one(int kind, uint64_t key, int i, int n, float* __restrict__ abc) {
+    float ai, bi, ci;
+    if (kind == 2 || kind == 3) {
+        ai = i == 0 ? 0.0f : (float)(-1.0 / 6.0);
+        ci = i == n - 1 ? 0.0f : (float)(-1.0 / 6.0);
+        bi = 1.0f;
+    } else {
+        gen_coef(key, i, n, kind, ai, bi, ci);
     }
-}
-
-hipError_t launch_generate_coef(int kind, uint64_t seed, uint64_t sys, int n, float* abc, hipStream_t stream) {
-    int blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(generate_coef_kernel, dim3(blocks), dim3(256), 0, stream, kind, seed, sys, n, abc);
-    return hipGetLastError();
+    abc[i] = ai;
+    abc[n + i] = bi;
+    abc[2 * (size_t)n + i] = ci;
 }
 
 // A flat grid over the concatenated values of every job: each thread finds
 // its job by binary search over the jobs' first indices.
-__global__ void generate_rhs_jobs_kernel(const RhsJob* __restrict__ jobs, int njobs, long long total,
-                                         float* __restrict__ base) {
+__global__ void generate_jobs_kernel(const GenJob* __restrict__ jobs, int njobs, long long total,
+                                     float* __restrict__ base) {
     for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += (long long)gridDim.x * blockDim.x) {
         int lo = 0, hi = njobs - 1;  // the last job with first <= g
@@ -156,22 +145,25 @@ __global__ void generate_rhs_jobs_kernel(const RhsJob* __restrict__ jobs, int nj
             if (jobs[mid].first <= g) lo = mid;
             else hi = mid - 1;
         }
-        const RhsJob& jb = jobs[lo];
+        const GenJob& jb = jobs[lo];
         const uint64_t e = (uint64_t)(g - jb.first);  // value index inside the job
         const uint64_t key = mix64(jb.seed + kGolden * (jb.sys + 1));
-        const uint64_t jj = (uint64_t)jb.j0 + (e >> jb.lg_n);
-        const uint64_t idx = (e & ((1ULL << jb.lg_n) - 1));
-        base[jb.dst + (long long)e] = u_pm1(draw24(key, kStMulti, (jj << jb.lg_n) + idx));
+        float* dst = base + jb.dst;
+        if (jb.coef_kind == 0) {
+            const uint64_t jj = (uint64_t)jb.j0 + (e >> jb.lg_n);
+            const uint64_t idx = (e & ((1ULL << jb.lg_n) - 1));
+            dst[e] = u_pm1(draw24(key, kStMulti, (jj << jb.lg_n) + idx));
+        } else {
+            coef_one(jb.coef_kind - 1, key, (int)e, 1 << jb.lg_n, dst);
+        }
     }
 }
 
-hipError_t launch_generate_rhs_jobs(const RhsJob* jobs, int njobs, long long total, float* base,
-                                    hipStream_t stream) {
+hipError_t launch_generate_jobs(const GenJob* jobs, int njobs, long long total, float* base, hipStream_t stream) {
     if (njobs == 0 || total == 0) return hipSuccess;
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(generate_rhs_jobs_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, jobs, njobs, total,
-                       base);
+    hipLaunchKernelGGL(generate_jobs_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, jobs, njobs, total, base);
     return hipGetLastError();
 }
 

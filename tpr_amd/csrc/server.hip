@@ -12,6 +12,7 @@
 //      batch has room for their right-hand sides; the batch's tables are
 //      built on the host and copied through a pinned staging buffer in one
 //      transfer, and on the slot's own stream one launch generates the
+//      coefficients of generated matrices that this batch uses first and the
 //      synthetic right-hand sides, one persistent serve kernel solves every
 //      class within a workgroup, and one split launch set per larger class
 //      solves the rest;
@@ -102,14 +103,24 @@ private:
     std::map<size_t, size_t> free_;  // offset -> size
 };
 
+// Generation state of a generated matrix: its coefficients are written by the
+// first batch that uses it, in that batch's generation launch.
+enum { kGenPending = 0, kGenRunning = 1, kGenDone = 2 };
+
 struct Matrix {
     int n = 0;
     size_t off = 0, bytes = 0;  // packed a | b | c in the arena
     uint64_t sys = 0;           // generator id (right-hand sides generated for it use it)
     int refs = 0;               // queued or in-flight requests
     bool released = false;
+    // uploaded matrices: the upload's event (complete when the upload returns)
     bool ready_known = false;   // the ready event has been seen complete
     hipEvent_t ready = nullptr;
+    // generated matrices (ready == nullptr)
+    int gen_kind = 0;
+    uint64_t gen_seed = 0;
+    int gen_state = kGenDone;
+    int gen_slot = -1;          // the slot whose batch generates it (kGenRunning)
 };
 
 enum { kQueued = 0, kOnDevice = 1, kDone = 2, kFailed = 3 };
@@ -134,6 +145,8 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t gen_done = nullptr;     // after the batch's generation launch
+    std::vector<long long> gen_mats;   // matrices this batch generates
     void* host_stage = nullptr;  // pinned: the batch's packed tables
     void* dev_table = nullptr;   // device: the same tables, then the work counter
     bool busy = false;
@@ -156,6 +169,10 @@ struct tprh_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<long long> queue;
+    long long queued_systems = 0;  // right-hand sides of the queued requests
+    // bumped (under mu) by every call that may let a queued request in: a
+    // submit, a release that frees arena memory, the destructor
+    std::atomic<unsigned long long> wake{0};
     std::unordered_map<long long, Request> reqs;
     std::unordered_map<long long, Matrix> mats;
     std::vector<long long> completed;
@@ -203,6 +220,14 @@ void drop_request_ref_locked(tprh_server* sv, Request& r) {
 // Retires the batch of a slot whose event has fired (or failed).
 void retire_locked(tprh_server* sv, Slot& sl, hipError_t e) {
     const double t = sv->now();
+    // before the requests drop their references: the matrices are alive
+    for (long long mid : sl.gen_mats) {
+        auto it = sv->mats.find(mid);
+        if (it == sv->mats.end()) continue;
+        it->second.gen_state = e == hipSuccess ? kGenDone : kGenPending;
+        it->second.gen_slot = -1;
+    }
+    sl.gen_mats.clear();
     for (long long id : sl.reqs) {
         auto it = sv->reqs.find(id);
         if (it == sv->reqs.end()) continue;
@@ -253,29 +278,61 @@ struct ClassKey {
     bool operator<(const ClassKey& o) const { return n > o.n; }  // largest systems first
 };
 
-// Builds and launches one batch on slot `sl` from the requests in `ids`
-// (already holding their arena blocks).  Every table of the batch (work
+// One admitted request as the launch sees it: a copy taken under the server
+// lock, so that the launch itself can run without it (other threads may add
+// requests and matrices meanwhile; this request's block, its matrix and the
+// matrix's ready event stay valid because the request is on the device and
+// holds a reference to the matrix).
+struct Staged {
+    Request r;
+    size_t mat_off;
+    uint64_t mat_sys;
+    hipEvent_t ready;     // uploaded matrix not yet seen complete: wait for it
+    int wait_slot;        // generated by another slot's running batch: wait for its generation
+    int gen_kind;         // >= 0: this batch generates the matrix (first request to use it)
+    uint64_t gen_seed;
+};
+
+// Builds and launches one batch on slot `sl` (busy, so this thread owns its
+// stream and tables) from the staged requests; called without the lock.  Every table of the batch (work
 // items, serve-kernel entries, split entries, right-hand-side jobs) is
 // packed into the slot's pinned staging buffer and copied with one
 // transfer; then, on the slot's stream: caller right-hand sides are copied
-// in, synthetic ones generated (one launch), the serve kernel runs for every
+// in, synthetic ones and the coefficients of matrices first used here
+// generated (one launch, then the gen_done event that batches on other slots
+// using those matrices wait for), the serve kernel runs for every
 // class within a workgroup, one split launch set per larger class, and the
 // answers the caller asked for are copied out.  Returns the first HIP error.
-hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long long>& ids) {
+hipError_t launch_batch(tprh_server* sv, Slot& sl, const std::vector<Staged>& items) {
     float* basef = reinterpret_cast<float*>(sv->base);
     const size_t fsz = sizeof(float);
     hipError_t e = hipSuccess;
     std::map<ClassKey, std::vector<tprh::ServeSys>> groups;          // classes within a workgroup
     std::map<int, std::pair<int, int>> shape_of;                      // n -> (E, L)
     std::map<int, std::vector<tprh::ServeSplitSys>> split_groups;     // classes above a workgroup
-    std::vector<tprh::RhsJob> jobs;
-    long long rhs_total = 0;
-    for (long long id : ids) {
-        Request& r = sv->reqs[id];
-        Matrix& m = sv->mats[r.mat];
-        if (!m.ready_known) {
-            if (hipEventQuery(m.ready) == hipSuccess) m.ready_known = true;
-            else if ((e = hipStreamWaitEvent(sl.stream, m.ready, 0)) != hipSuccess) return e;
+    std::vector<tprh::GenJob> jobs;
+    long long gen_total = 0;
+    bool generates = false;
+    unsigned waited = 0;  // slots already waited for
+    for (const Staged& it : items) {
+        const Request& r = it.r;
+        if (it.ready && (e = hipStreamWaitEvent(sl.stream, it.ready, 0)) != hipSuccess) return e;
+        if (it.wait_slot >= 0 && !(waited >> it.wait_slot & 1u)) {
+            if ((e = hipStreamWaitEvent(sl.stream, sv->slots[it.wait_slot].gen_done, 0)) != hipSuccess) return e;
+            waited |= 1u << it.wait_slot;
+        }
+        if (it.gen_kind >= 0) {
+            tprh::GenJob jb{};
+            jb.seed = it.gen_seed;
+            jb.sys = it.mat_sys;
+            jb.dst = (long long)(it.mat_off / fsz);
+            jb.first = gen_total;
+            jb.nrhs = 1;
+            jb.lg_n = __builtin_ctz((unsigned)r.n);
+            jb.coef_kind = it.gen_kind + 1;
+            jobs.push_back(jb);
+            gen_total += r.n;
+            generates = true;
         }
         const size_t nn = (size_t)r.n;
         if (r.d_src) {
@@ -283,23 +340,23 @@ hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long
                 hipSuccess)
                 return e;
         } else {
-            tprh::RhsJob jb{};
+            tprh::GenJob jb{};
             jb.seed = r.rhs_seed;
-            jb.sys = m.sys;
+            jb.sys = it.mat_sys;
             jb.dst = (long long)(r.off / fsz);
-            jb.first = rhs_total;
+            jb.first = gen_total;
             jb.j0 = r.j0;
             jb.nrhs = r.nrhs;
             jb.lg_n = __builtin_ctz((unsigned)r.n);
             jobs.push_back(jb);
-            rhs_total += (long long)r.nrhs * r.n;
+            gen_total += (long long)r.nrhs * r.n;
         }
         shape_of[r.n] = {r.E, r.L};
         if (!r.split) {
             auto& g = groups[ClassKey{r.n}];
             for (int j = 0; j < r.nrhs; j++) {
                 tprh::ServeSys ss;
-                ss.coef = (long long)(m.off / fsz);
+                ss.coef = (long long)(it.mat_off / fsz);
                 ss.rhs = (long long)(r.off / fsz + (size_t)j * nn);
                 ss.x = (long long)(r.x_off / fsz + (size_t)j * nn);
                 g.push_back(ss);
@@ -311,7 +368,7 @@ hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long
             auto& g = split_groups[r.n];
             for (int j = 0; j < r.nrhs; j++) {
                 tprh::ServeSplitSys ss;
-                ss.coef = (long long)(m.off / fsz);
+                ss.coef = (long long)(it.mat_off / fsz);
                 ss.rhs = (long long)(r.off / fsz + (size_t)j * nn);
                 ss.x = (long long)(r.x_off / fsz + (size_t)j * nn);
                 ss.rows = (long long)(rows0 + (size_t)j * ms * tprh::kSplitRowWords);
@@ -355,12 +412,14 @@ hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long
     std::vector<std::pair<size_t, size_t>> split_at;  // (offset, count) per split class, in map order
     for (auto& kv : split_groups)
         split_at.push_back({put(kv.second.data(), sizeof(tprh::ServeSplitSys) * kv.second.size()), kv.second.size()});
-    const size_t o_jobs = put(jobs.data(), sizeof(tprh::RhsJob) * jobs.size());
+    const size_t o_jobs = put(jobs.data(), sizeof(tprh::GenJob) * jobs.size());
     if (at > 0 && (e = hipMemcpyAsync(dt, hs, at, hipMemcpyHostToDevice, sl.stream)) != hipSuccess) return e;
     if (!jobs.empty() &&
-        (e = tprh::launch_generate_rhs_jobs(reinterpret_cast<const tprh::RhsJob*>(dt + o_jobs), (int)jobs.size(),
-                                            rhs_total, basef, sl.stream)) != hipSuccess)
+        (e = tprh::launch_generate_jobs(reinterpret_cast<const tprh::GenJob*>(dt + o_jobs), (int)jobs.size(),
+                                        gen_total, basef, sl.stream)) != hipSuccess)
         return e;
+    // batches on other slots that use a matrix generated here wait for this
+    if (generates && (e = hipEventRecord(sl.gen_done, sl.stream)) != hipSuccess) return e;
     if (!tasks.empty()) {
         tprh::ServeLaunch p;
         p.tasks = reinterpret_cast<const tprh::UnevenTask*>(dt + o_tasks);
@@ -384,8 +443,8 @@ hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long
             return e;
         k++;
     }
-    for (long long id : ids) {
-        Request& r = sv->reqs[id];
+    for (const Staged& it : items) {
+        const Request& r = it.r;
         if (r.x_dst &&
             (e = hipMemcpyAsync(r.x_dst, basef + r.x_off / fsz, fsz * (size_t)r.n * r.nrhs, hipMemcpyDefault,
                                 sl.stream)) != hipSuccess)
@@ -394,26 +453,51 @@ hipError_t launch_batch_locked(tprh_server* sv, Slot& sl, const std::vector<long
     return hipEventRecord(sl.done, sl.stream);
 }
 
+// The worker polls its in-flight batches without the lock (a slot's state
+// changes only on this thread) and takes the lock only when a batch has
+// finished, when a slot is free and a submit or a release has happened since
+// its last admission pass (the wake counter), or when it is about to sleep;
+// so that client calls are not starved while batches run.
 void worker_main(tprh_server* sv) {
     (void)hipSetDevice(sv->device);
-    std::unique_lock<std::mutex> lk(sv->mu);
+    const int ns = (int)sv->slots.size();
+    std::vector<std::pair<int, hipError_t>> fired;
+    fired.reserve(ns);
+    unsigned long long seen = ~0ull;  // wake counter at the last admission pass
+    bool recheck = true;              // admission must run again without a wake
     for (;;) {
         // 1. completions
-        int inflight = 0;
-        for (Slot& sl : sv->slots) {
-            if (!sl.busy) continue;
-            hipError_t q = hipEventQuery(sl.done);
-            if (q == hipErrorNotReady) {
-                inflight++;
+        int inflight = 0, nfree = 0;
+        fired.clear();
+        for (int k = 0; k < ns; k++) {
+            Slot& sl = sv->slots[k];
+            if (!sl.busy) {
+                nfree++;
                 continue;
             }
-            retire_locked(sv, sl, q);
-            sv->cv_done.notify_all();
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) inflight++;
+            else fired.push_back({k, q});
         }
+        const unsigned long long wake = sv->wake.load(std::memory_order_acquire);
+        if (fired.empty() && inflight > 0 && !recheck && (nfree == 0 || wake == seen)) {
+            std::this_thread::yield();
+            continue;
+        }
+        std::unique_lock<std::mutex> lk(sv->mu);
+        seen = wake;
+        recheck = false;
+        for (const auto& f : fired) retire_locked(sv, sv->slots[f.first], f.second);
+        if (!fired.empty()) sv->cv_done.notify_all();
         // 2. admission and launch
         bool launched = false;
         for (Slot& sl : sv->slots) {
             if (sl.busy || sv->queue.empty()) continue;
+            if (inflight > 0 && sv->queued_systems < sv->cfg.min_batch_systems &&
+                1e6 * (sv->now() - sv->reqs[sv->queue.front()].t_submit) < sv->cfg.max_wait_us) {
+                recheck = true;
+                break;  // batches are running: let this one grow
+            }
             std::vector<long long> batch;
             long long systems = 0, equations = 0;
             while (!sv->queue.empty()) {
@@ -428,6 +512,7 @@ void worker_main(tprh_server* sv) {
                 r.x_off = off + align_up(xb);
                 r.ws_off = r.x_off + align_up(xb);
                 sv->queue.pop_front();
+                sv->queued_systems -= r.nrhs;
                 batch.push_back(id);
                 systems += r.nrhs;
                 equations += (long long)r.nrhs * r.n;
@@ -443,7 +528,30 @@ void worker_main(tprh_server* sv) {
             sl.systems = systems;
             sl.equations = equations;
             sl.busy = true;
-            hipError_t e = launch_batch_locked(sv, sl, batch);
+            const int slot = (int)(&sl - sv->slots.data());
+            std::vector<Staged> items;
+            items.reserve(batch.size());
+            for (long long id : batch) {
+                const Request& r = sv->reqs[id];
+                Matrix& m = sv->mats[r.mat];
+                Staged it{r, m.off, m.sys, nullptr, -1, -1, 0};
+                if (m.ready) {
+                    if (!m.ready_known && hipEventQuery(m.ready) == hipSuccess) m.ready_known = true;
+                    if (!m.ready_known) it.ready = m.ready;
+                } else if (m.gen_state == kGenPending) {
+                    m.gen_state = kGenRunning;
+                    m.gen_slot = slot;
+                    sl.gen_mats.push_back(r.mat);
+                    it.gen_kind = m.gen_kind;
+                    it.gen_seed = m.gen_seed;
+                } else if (m.gen_state == kGenRunning && m.gen_slot != slot) {
+                    it.wait_slot = m.gen_slot;
+                }
+                items.push_back(it);
+            }
+            lk.unlock();  // the HIP calls of the launch run without the lock
+            hipError_t e = launch_batch(sv, sl, items);
+            lk.lock();
             if (e != hipSuccess) {
                 (void)hipStreamSynchronize(sl.stream);
                 retire_locked(sv, sl, e);
@@ -469,18 +577,16 @@ void worker_main(tprh_server* sv) {
                 sv->st.failed++;
             }
             sv->queue.clear();
+            sv->queued_systems = 0;
             sv->worker_error = "server destroyed before the request could be admitted";
             sv->cv_done.notify_all();
         }
         sv->st.queued = (int)sv->queue.size();
         if (sv->stop && sv->queue.empty() && inflight == 0) break;
-        // 3. idle or poll
+        // 3. idle: sleep until a submit, a release or the destructor
         if (inflight == 0 && !launched) {
             sv->cv_work.wait_for(lk, std::chrono::milliseconds(50));
-        } else if (!launched) {
-            lk.unlock();
-            std::this_thread::yield();
-            lk.lock();
+            recheck = true;
         }
     }
 }
@@ -511,6 +617,8 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
     if (cfg->max_batch_systems < 1 || cfg->max_batch_systems > (1 << 20))
         return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "max_batch_systems must be 1..2^20");
     if (cfg->hbm_budget < kAlign) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "hbm_budget too small");
+    if (cfg->min_batch_systems < 0 || !(cfg->max_wait_us >= 0.0))
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "batching window must be non-negative");
     if (cfg->schedule) {
         // every row must name a shape the serve kernel has (or be zero)
         for (int lg = 4; lg < 32; lg++) {
@@ -541,6 +649,7 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
         for (Slot& sl : sv->slots) {
             if (sl.stream) (void)hipStreamDestroy(sl.stream);
             if (sl.done) (void)hipEventDestroy(sl.done);
+            if (sl.gen_done) (void)hipEventDestroy(sl.gen_done);
             if (sl.host_stage) (void)hipHostFree(sl.host_stage);
             if (sl.dev_table) (void)hipFree(sl.dev_table);
         }
@@ -564,12 +673,13 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
     sv->table_bytes = align_up(sizeof(tprh::UnevenTask) * (size_t)cfg->max_batch_systems) +
                       align_up(sizeof(tprh::ServeSys) * (size_t)cfg->max_batch_systems) +
                       align_up(sizeof(tprh::ServeSplitSys) * (size_t)cfg->max_batch_systems) + 32 * kAlign +
-                      align_up(sizeof(tprh::RhsJob) * (size_t)cfg->max_batch_systems);
+                      align_up(sizeof(tprh::GenJob) * 2 * (size_t)cfg->max_batch_systems);
     sv->slots.resize(cfg->max_inflight);
     for (Slot& sl : sv->slots) {
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
             return bail(e, "hipStreamCreate");
-        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
+        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&sl.gen_done, hipEventDisableTiming)) != hipSuccess)
             return bail(e, "hipEventCreate");
         if ((e = hipHostMalloc(&sl.host_stage, sv->table_bytes)) != hipSuccess) return bail(e, "hipHostMalloc");
         if ((e = hipMalloc(&sl.dev_table, sv->table_bytes + kAlign)) != hipSuccess) return bail(e, "hipMalloc");
@@ -595,6 +705,7 @@ tprh_status tprh_server_destroy(tprh_server* sv) {
     {
         std::lock_guard<std::mutex> g(sv->mu);
         sv->stop = true;
+        sv->wake++;
     }
     sv->cv_work.notify_all();
     if (sv->worker.joinable()) sv->worker.join();
@@ -604,6 +715,7 @@ tprh_status tprh_server_destroy(tprh_server* sv) {
     for (Slot& sl : sv->slots) {
         (void)hipStreamDestroy(sl.stream);
         (void)hipEventDestroy(sl.done);
+        (void)hipEventDestroy(sl.gen_done);
         (void)hipHostFree(sl.host_stage);
         (void)hipFree(sl.dev_table);
     }
@@ -613,52 +725,63 @@ tprh_status tprh_server_destroy(tprh_server* sv) {
     return TPRH_SUCCESS;
 }
 
-static tprh_status new_matrix(tprh_server* sv, int n, long long* id, Matrix** out) {
-    if (!id) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+// Reserves the arena block and the record of a new matrix, under the lock.
+// An uploaded matrix (ev != nullptr) is then filled by the caller without the
+// lock: its id is not yet known to any other call, so nothing else touches
+// the record.  A generated one is filled by the first batch that uses it.
+static tprh_status new_matrix(tprh_server* sv, int n, uint64_t sys, hipEvent_t ev, int gen_kind, uint64_t gen_seed,
+                              long long* id, size_t* off_out) {
+    std::lock_guard<std::mutex> g(sv->mu);
     int s, E, L;
     bool split;
     tprh_status st = validate_n(sv, n, &s, &E, &L, &split);
     if (st != TPRH_SUCCESS) return st;
     const size_t bytes = sizeof(float) * 3 * (size_t)n;
-    size_t off;
+    size_t off = 0;
     if (!sv->arena.alloc(bytes, &off))
         return tprh::set_error(TPRH_ERROR_BUDGET, "no room for the matrix in the server's memory budget");
-    hipEvent_t ev;
-    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        sv->arena.release(off, bytes);
-        return tprh::set_hip_error(e, "hipEventCreate");
-    }
     const long long mid = sv->next_mat++;
     Matrix& m = sv->mats[mid];
     m.n = n;
     m.off = off;
     m.bytes = bytes;
+    m.sys = sys;
     m.ready = ev;
+    if (!ev) {
+        m.gen_kind = gen_kind;
+        m.gen_seed = gen_seed;
+        m.gen_state = kGenPending;
+    }
     *id = mid;
-    *out = &m;
+    *off_out = off;
     return TPRH_SUCCESS;
 }
 
 tprh_status tprh_server_matrix_upload(tprh_server* sv, int n, const float* a, const float* b, const float* c,
                                       long long* matrix) {
-    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    if (!sv || !matrix) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
     if (!a || !b || !c) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null array pointer");
-    std::lock_guard<std::mutex> g(sv->mu);
-    Matrix* m;
-    tprh_status st = new_matrix(sv, n, matrix, &m);
-    if (st != TPRH_SUCCESS) return st;
-    float* dst = reinterpret_cast<float*>(sv->base + m->off);
+    hipEvent_t ev;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return tprh::set_hip_error(e, "hipEventCreate");
+    size_t off;
+    tprh_status st = new_matrix(sv, n, 0, ev, 0, 0, matrix, &off);
+    if (st != TPRH_SUCCESS) {
+        (void)hipEventDestroy(ev);
+        return st;
+    }
+    float* dst = reinterpret_cast<float*>(sv->base + off);
     const size_t nb = sizeof(float) * (size_t)n;
-    hipError_t e;
     if ((e = hipMemcpyAsync(dst, a, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
         (e = hipMemcpyAsync(dst + n, b, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
         (e = hipMemcpyAsync(dst + 2 * (size_t)n, c, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
-        (e = hipEventRecord(m->ready, sv->upload)) != hipSuccess || (e = hipEventSynchronize(m->ready)) != hipSuccess) {
+        (e = hipEventRecord(ev, sv->upload)) != hipSuccess || (e = hipEventSynchronize(ev)) != hipSuccess) {
+        std::lock_guard<std::mutex> g(sv->mu);
         free_matrix_locked(sv, *matrix);
         return tprh::set_hip_error(e, "matrix upload");
     }
-    m->ready_known = true;
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->mats[*matrix].ready_known = true;
     return TPRH_SUCCESS;
 }
 
@@ -666,20 +789,9 @@ tprh_status tprh_server_matrix_generate(tprh_server* sv, int n, tprh_gen kind, u
                                         long long* matrix) {
     if (kind < TPRH_GEN_DOMINANT || kind > TPRH_GEN_RANDOM_RHS)
         return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "bad kind");
-    if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
-    std::lock_guard<std::mutex> g(sv->mu);
-    Matrix* m;
-    tprh_status st = new_matrix(sv, n, matrix, &m);
-    if (st != TPRH_SUCCESS) return st;
-    m->sys = sys;
-    float* dst = reinterpret_cast<float*>(sv->base + m->off);
-    hipError_t e;
-    if ((e = tprh::launch_generate_coef((int)kind, seed, sys, n, dst, sv->upload)) != hipSuccess ||
-        (e = hipEventRecord(m->ready, sv->upload)) != hipSuccess) {
-        free_matrix_locked(sv, *matrix);
-        return tprh::set_hip_error(e, "matrix generation");
-    }
-    return TPRH_SUCCESS;
+    if (!sv || !matrix) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    size_t off;
+    return new_matrix(sv, n, sys, nullptr, (int)kind, seed, matrix, &off);
 }
 
 tprh_status tprh_server_matrix_release(tprh_server* sv, long long matrix) {
@@ -691,6 +803,7 @@ tprh_status tprh_server_matrix_release(tprh_server* sv, long long matrix) {
     it->second.released = true;
     if (it->second.refs == 0) {
         free_matrix_locked(sv, matrix);
+        sv->wake++;
         sv->cv_work.notify_one();  // queued requests may fit now
     }
     return TPRH_SUCCESS;
@@ -725,6 +838,8 @@ tprh_status tprh_server_submit(tprh_server* sv, long long matrix, int nrhs, cons
     const long long id = sv->next_req++;
     sv->reqs[id] = r;
     sv->queue.push_back(id);
+    sv->queued_systems += nrhs;
+    sv->wake++;
     m.refs++;
     sv->st.submitted++;
     *request = id;
@@ -803,6 +918,7 @@ tprh_status tprh_server_request_release(tprh_server* sv, long long request) {
         return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "the request has not completed");
     if (r.holds_block) {
         sv->arena.release(r.off, r.bytes);
+        sv->wake++;
         sv->cv_work.notify_one();  // queued requests may fit now
     }
     sv->reqs.erase(it);
