@@ -80,6 +80,14 @@ class Server {
         ids.resize(cnt);
         return ids;
     }
+    // sleeps until a request completes (or timeout_s passes: empty result)
+    std::vector<long long> wait_any(double timeout_s, int max = 4096) {
+        std::vector<long long> ids(max);
+        int cnt = 0;
+        check(tprh_server_wait_any(s_, ids.data(), max, &cnt, timeout_s), "tprh_server_wait_any");
+        ids.resize(cnt);
+        return ids;
+    }
     tprh_request_info info(long long r) {
         tprh_request_info inf;
         check(tprh_server_request_info(s_, r, &inf), "tprh_server_request_info");

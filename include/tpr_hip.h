@@ -276,6 +276,10 @@ tprh_status tprh_server_request_info(tprh_server *srv, long long request, tprh_r
 tprh_status tprh_server_wait(tprh_server *srv, long long request, double timeout_s);
 /* Ids of requests completed since the previous call (at most max). */
 tprh_status tprh_server_poll(tprh_server *srv, long long *ids, int max, int *count);
+/* As tprh_server_poll, but sleeps until at least one request has completed
+ * or timeout_s passes (then *count is 0).  Lets a client wait for whichever
+ * request finishes first without spinning on the server. */
+tprh_status tprh_server_wait_any(tprh_server *srv, long long *ids, int max, int *count, double timeout_s);
 /* Copies a completed request's retained answers (nrhs * n floats) to dst
  * (host or device); returns when the copy is done. */
 tprh_status tprh_server_copy_result(tprh_server *srv, long long request, float *dst);

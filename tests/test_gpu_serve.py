@@ -184,6 +184,44 @@ def test_serve_shared_generated_matrices(kind):
         assert st["failed"] == 0 and st["bytes_in_use"] == 0
 
 
+def test_serve_batching_window():
+    """With a batching window (a batch waits for min_batch_systems queued
+    right-hand sides or max_wait_us while others run) every request still
+    completes, bit-identical to the oracle, in fewer batches than requests."""
+    n, nrhs, nreq = 1024, 2, 48
+    say("serve batching window")
+    with Server(64 << 20, arith="exact", max_inflight=2, min_batch_systems=16, max_wait_us=500.0) as srv:
+        m = srv.matrix_generate(n, seed=SEED, sys=4100)
+        rids = [srv.submit(m, nrhs, rhs_seed=SEED, j0=nrhs * q) for q in range(nreq)]
+        for q, rid in enumerate(rids):
+            srv.wait(rid)
+            ref = oracle_answers("ptpr", 4100, n, nrhs, j0=nrhs * q)
+            assert bits_equal(srv.result(rid).cpu().numpy(), ref), q
+            srv.release(rid)
+        srv.matrix_release(m)
+        st = srv.stats()
+    assert st["completed"] == nreq and st["failed"] == 0 and st["batches"] < nreq
+
+
+def test_serve_wait_any():
+    """wait_any hands out every completed id exactly once, in whatever order
+    the batches finish, and returns empty after its timeout when none is left."""
+    say("serve wait_any")
+    with Server(64 << 20, max_inflight=4, max_batch_systems=8) as srv:
+        m = srv.matrix_generate(4096, seed=SEED, sys=4200)
+        rids = {srv.submit(m, 4, rhs_seed=SEED, j0=4 * q) for q in range(24)}
+        seen = []
+        while len(seen) < len(rids):
+            got = srv.poll(timeout=30.0)
+            assert got, "wait_any timed out with requests outstanding"
+            seen += got
+        assert sorted(seen) == sorted(rids)
+        assert srv.poll(timeout=0.01) == []
+        for r in rids:
+            srv.release(r)
+        srv.matrix_release(m)
+
+
 def test_serve_errors():
     with Server(16 << 20) as srv:
         with pytest.raises(_lib.TprError):

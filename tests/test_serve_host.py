@@ -16,6 +16,8 @@ def _create(**kw):
     cfg.hbm_budget = kw.get("budget", 1 << 20)
     cfg.max_inflight = kw.get("inflight", 2)
     cfg.max_batch_systems = kw.get("batch", 64)
+    cfg.min_batch_systems = kw.get("min_batch", 0)
+    cfg.max_wait_us = kw.get("max_wait", 0.0)
     tab = kw.get("schedule")
     cfg.schedule = tab.ctypes.data if tab is not None else None
     h = ctypes.c_void_p()
@@ -24,7 +26,8 @@ def _create(**kw):
 
 
 @pytest.mark.parametrize("kw", [dict(method=_lib.PCR_M), dict(arith=9), dict(inflight=0), dict(inflight=17),
-                                dict(batch=0), dict(budget=16)])
+                                dict(batch=0), dict(budget=16), dict(min_batch=-1), dict(max_wait=-1.0),
+                                dict(max_wait=float("nan"))])
 def test_bad_config_refused_before_device(kw):
     st, h = _create(**kw)
     assert st == _lib.ERROR_INVALID_VALUE
@@ -46,6 +49,8 @@ def test_null_handles_refused():
     assert L.tprh_server_matrix_release(None, 1) == _lib.ERROR_INVALID_VALUE
     cnt = ctypes.c_int()
     assert L.tprh_server_poll(None, None, 0, ctypes.byref(cnt)) == _lib.ERROR_INVALID_VALUE
+    buf = (ctypes.c_longlong * 4)()
+    assert L.tprh_server_wait_any(None, buf, 4, ctypes.byref(cnt), 0.0) == _lib.ERROR_INVALID_VALUE
     assert L.tprh_server_destroy(None) == _lib.SUCCESS
     assert L.tprh_status_string(_lib.ERROR_TIMEOUT) == b"timed out"
     assert L.tprh_status_string(_lib.ERROR_BUDGET) == b"memory budget exceeded"

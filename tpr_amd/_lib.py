@@ -112,6 +112,7 @@ SIGNATURES.update({
     "tprh_server_request_info": (_i, [_vp, _ll, ctypes.POINTER(RequestInfo)]),
     "tprh_server_wait": (_i, [_vp, _ll, ctypes.c_double]),
     "tprh_server_poll": (_i, [_vp, ctypes.POINTER(_ll), _i, ctypes.POINTER(_i)]),
+    "tprh_server_wait_any": (_i, [_vp, ctypes.POINTER(_ll), _i, ctypes.POINTER(_i), ctypes.c_double]),
     "tprh_server_copy_result": (_i, [_vp, _ll, _vp]),
     "tprh_server_request_release": (_i, [_vp, _ll]),
     "tprh_server_now": (ctypes.c_double, [_vp]),

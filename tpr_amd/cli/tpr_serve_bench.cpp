@@ -12,11 +12,13 @@
 // (seed, request id) and resident in the server's arena, and 1..H right-hand
 // sides generated on the device (d ~ U(-1,1)); the answers stay in the arena
 // until the client sees the request complete, then request and matrix are
-// released.  Latency is submit to completion on the server's clock.
+// released.  Latency is submit to completion on the server's clock, split
+// into the wait in the queue (submit to launch) and the time on the device
+// (launch to the completion the server sees).
 // C client threads submit (request k belongs to client k mod C; in the closed
 // loop each keeps at most Q / C outstanding) and one collector thread takes
-// the completions, records latencies and releases; the server's calls are
-// thread-safe.  Prints one JSON line.
+// the completions (sleeping in tprh_server_wait_any), records latencies and
+// releases; the server's calls are thread-safe.  Prints one JSON line.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -25,11 +27,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "tpr_hip.h"
@@ -153,18 +153,31 @@ int main(int argc, char** argv) {
     std::vector<long long> polled(4096);
     int cnt = 0;
     check(tprh_server_poll(srv, polled.data(), (int)polled.size(), &cnt), "poll");  // drop the warm-up ids
-
-    struct Owner {
-        long long matrix;
-        int client;
-    };
-    std::mutex mu;                                // guards owner
-    std::unordered_map<long long, Owner> owner;   // request -> (matrix, client)
+    // request ids are handed out in sequence and only this program submits:
+    // the timed requests get ids first_id .. first_id + nreq - 1
+    long long first_id = 0;
+    {
+        long long m;
+        check(tprh_server_matrix_generate(srv, 1 << a.min_log2, TPRH_GEN_DOMINANT, a.seed, 1ull << 41, &m), "matrix");
+        check(tprh_server_submit(srv, m, 1, nullptr, a.seed, 0, nullptr, &first_id), "submit");
+        check(tprh_server_wait(srv, first_id, 60.0), "wait");
+        check(tprh_server_request_release(srv, first_id), "release");
+        check(tprh_server_matrix_release(srv, m), "matrix release");
+        check(tprh_server_poll(srv, polled.data(), (int)polled.size(), &cnt), "poll");
+        first_id++;
+    }
+    // owner of each request, written by its client after submit returns
+    // (matrix + 1, 0 while unset) and read by the collector
+    std::vector<std::atomic<long long>> owner_mat(nreq);
+    std::vector<int> owner_client(nreq);
+    for (auto& o : owner_mat) o.store(0);
     std::vector<std::atomic<int>> outstanding(a.clients);
     for (auto& o : outstanding) o.store(0);
     const int per_client = a.outstanding / a.clients;
-    std::vector<double> lat;
+    std::vector<double> lat, qwait, dev;  // submit->done, submit->launch, launch->done
     lat.reserve(nreq);
+    qwait.reserve(nreq);
+    dev.reserve(nreq);
     std::atomic<long long> late_ns{0};
     std::atomic<bool> failed{false};
     const double t0 = tprh_server_now(srv);
@@ -182,14 +195,14 @@ int main(int argc, char** argv) {
             }
             long long m, r;
             check(tprh_server_matrix_generate(srv, n[k], TPRH_GEN_DOMINANT, a.seed, (uint64_t)k, &m), "matrix");
-            {
-                // the collector looks a completed id up under the same lock,
-                // so it never sees a request before its owner is recorded
-                std::lock_guard<std::mutex> g(mu);
-                check(tprh_server_submit(srv, m, rhs[k], nullptr, a.seed, 0, nullptr, &r), "submit");
-                owner[r] = Owner{m, c};
-                outstanding[c]++;
+            outstanding[c]++;
+            check(tprh_server_submit(srv, m, rhs[k], nullptr, a.seed, 0, nullptr, &r), "submit");
+            if (r < first_id || r >= first_id + nreq) {
+                std::fprintf(stderr, "unexpected request id %lld\n", r);
+                std::exit(1);
             }
+            owner_client[r - first_id] = c;
+            owner_mat[r - first_id].store(m + 1, std::memory_order_release);
         }
     };
     auto collector = [&]() {
@@ -197,16 +210,14 @@ int main(int argc, char** argv) {
         long long done = 0;
         while (done < nreq && !failed.load()) {
             int got = 0;
-            check(tprh_server_poll(srv, ids.data(), (int)ids.size(), &got), "poll");
+            check(tprh_server_wait_any(srv, ids.data(), (int)ids.size(), &got, 0.01), "wait_any");
             for (int i = 0; i < got; i++) {
                 const long long r = ids[i];
-                Owner o;
-                {
-                    std::lock_guard<std::mutex> g(mu);
-                    auto it = owner.find(r);
-                    o = it->second;
-                    owner.erase(it);
-                }
+                // the request may complete before its client has recorded it
+                long long m1;
+                while ((m1 = owner_mat[r - first_id].load(std::memory_order_acquire)) == 0) std::this_thread::yield();
+                const long long matrix = m1 - 1;
+                const int owner = owner_client[r - first_id];
                 tprh_request_info inf;
                 check(tprh_server_request_info(srv, r, &inf), "info");
                 if (inf.state != 2) {
@@ -215,12 +226,13 @@ int main(int argc, char** argv) {
                     return;
                 }
                 lat.push_back(inf.t_done - inf.t_submit);
+                qwait.push_back(inf.t_launch - inf.t_submit);
+                dev.push_back(inf.t_done - inf.t_launch);
                 check(tprh_server_request_release(srv, r), "release");
-                check(tprh_server_matrix_release(srv, o.matrix), "matrix release");
-                outstanding[o.client]--;
+                check(tprh_server_matrix_release(srv, matrix), "matrix release");
+                outstanding[owner]--;
                 done++;
             }
-            if (got == 0) std::this_thread::yield();
         }
     };
     std::thread coll(collector);
@@ -242,12 +254,15 @@ int main(int argc, char** argv) {
     std::printf(
         "{\"mode\": \"%s\", \"rate\": %.1f, \"requests\": %lld, \"systems\": %lld, \"equations\": %lld, "
         "\"wall_s\": %.4f, \"requests_per_s\": %.1f, \"systems_per_s\": %.1f, \"GBps\": %.1f, "
-        "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"max\": %.4f}, \"max_arrival_lag_ms\": %.4f, "
+        "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"max\": %.4f}, "
+        "\"queued_ms\": {\"p50\": %.4f, \"p99\": %.4f}, \"on_device_ms\": {\"p50\": %.4f, \"p99\": %.4f}, "
+        "\"max_arrival_lag_ms\": %.4f, "
         "\"batches\": %lld, \"max_batch_systems_seen\": %lld, \"max_inflight_seen\": %d, \"peak_bytes\": %zu, "
         "\"outstanding\": %d, \"clients\": %d, \"min_batch\": %d, \"max_wait_us\": %.1f}\n",
         open ? "poisson" : "closed", a.rate, nreq, systems, equations, wall, (double)nreq / wall,
         (double)systems / wall, bytes / wall / 1e9, 1e3 * pct(lat, 50), 1e3 * pct(lat, 99),
-        1e3 * (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end())), 1e3 * late_max, st.batches,
+        1e3 * (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end())), 1e3 * pct(qwait, 50),
+        1e3 * pct(qwait, 99), 1e3 * pct(dev, 50), 1e3 * pct(dev, 99), 1e3 * late_max, st.batches,
         st.max_batch_systems_seen, st.max_inflight_seen, st.peak_bytes, open ? 0 : a.outstanding, a.clients,
         a.min_batch, a.max_wait_us);
     return 0;

@@ -683,6 +683,19 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
             return bail(e, "hipEventCreate");
         if ((e = hipHostMalloc(&sl.host_stage, sv->table_bytes)) != hipSuccess) return bail(e, "hipHostMalloc");
         if ((e = hipMalloc(&sl.dev_table, sv->table_bytes + kAlign)) != hipSuccess) return bail(e, "hipMalloc");
+        // one table copy of each size class, now: the runtime sets up its
+        // copy paths on first use (the first copy of a few tens of KB took
+        // about 6 ms of host time on MI355X), which would otherwise stall
+        // the first batches served
+        std::memset(sl.host_stage, 0, sv->table_bytes);
+        for (size_t bytes = kAlign;; bytes *= 8) {
+            bytes = std::min(bytes, sv->table_bytes);
+            if ((e = hipMemcpyAsync(sl.dev_table, sl.host_stage, bytes, hipMemcpyHostToDevice, sl.stream)) !=
+                hipSuccess)
+                return bail(e, "hipMemcpyAsync(tables)");
+            if (bytes == sv->table_bytes) break;
+        }
+        if ((e = hipStreamSynchronize(sl.stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
     }
     int cus = 0, occ = 0;
     sv->lds_bytes = kServeLds;
@@ -881,13 +894,27 @@ tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_
     }
 }
 
-tprh_status tprh_server_poll(tprh_server* sv, long long* ids, int max, int* count) {
-    if (!sv || !count || (max > 0 && !ids)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
-    std::lock_guard<std::mutex> g(sv->mu);
+static void take_completed_locked(tprh_server* sv, long long* ids, int max, int* count) {
     const int k = std::min<int>(max, (int)sv->completed.size());
     for (int i = 0; i < k; i++) ids[i] = sv->completed[i];
     sv->completed.erase(sv->completed.begin(), sv->completed.begin() + k);
     *count = k;
+}
+
+tprh_status tprh_server_poll(tprh_server* sv, long long* ids, int max, int* count) {
+    if (!sv || !count || (max > 0 && !ids)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    take_completed_locked(sv, ids, max, count);
+    return TPRH_SUCCESS;
+}
+
+tprh_status tprh_server_wait_any(tprh_server* sv, long long* ids, int max, int* count, double timeout_s) {
+    if (!sv || !count || max < 1 || !ids) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument or max < 1");
+    if (!(timeout_s >= 0.0)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "timeout must be >= 0");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    sv->cv_done.wait_until(lk, deadline, [sv] { return !sv->completed.empty(); });
+    take_completed_locked(sv, ids, max, count);
     return TPRH_SUCCESS;
 }
 

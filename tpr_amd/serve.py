@@ -104,10 +104,15 @@ class Server:
         _lib.call("tprh_server_copy_result", self._h, int(rid), ctypes.c_void_p(out.data_ptr()))
         return out.view(inf["nrhs"], inf["n"])
 
-    def poll(self, max_ids=4096):
+    def poll(self, max_ids=4096, timeout=None):
+        """Ids of requests completed since the last poll; with a timeout (s),
+        sleeps until at least one has completed or the timeout passes."""
         buf = (ctypes.c_longlong * max_ids)()
         cnt = ctypes.c_int()
-        _lib.call("tprh_server_poll", self._h, buf, int(max_ids), ctypes.byref(cnt))
+        if timeout is None:
+            _lib.call("tprh_server_poll", self._h, buf, int(max_ids), ctypes.byref(cnt))
+        else:
+            _lib.call("tprh_server_wait_any", self._h, buf, int(max_ids), ctypes.byref(cnt), float(timeout))
         return list(buf[:cnt.value])
 
     def release(self, rid):
