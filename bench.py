@@ -208,6 +208,22 @@ def vendor_leg(sets, host):
         return {"error": str(e)}
 
 
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the ones this process
+    may use, the CPU model and the OpenMP settings."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}
+
+
 def cpu_leg(args, host, x_gpu):
     import numpy as np
 
@@ -239,7 +255,8 @@ def cpu_leg(args, host, x_gpu):
     err_t = np.linalg.norm(xg - xt, axis=1) / np.linalg.norm(xt, axis=1)
     base = {"value": round(cpu_sys_s, 1), "unit": "systems/s", "cores": threads, "kind": kind,
             "sample": "%d of the %d systems (N=%d, S=%d), solved %d times in %.1f s, one %s solver per thread "
-                      "(OpenMP over systems)" % (n_sample, NSYS, N, S, reps, el, args.method.upper())}
+                      "(OpenMP over systems)" % (n_sample, NSYS, N, S, reps, el, args.method.upper()),
+            "host": host_info()}
     acc = {"rel_l2_vs_cpu_max": float(rel.max()), "rel_l2_vs_cpu_mean": float(rel.mean()),
            "bit_identical_to_cpu": bitexact, "rel_l2_vs_truth_max": float(err_t.max()), "cpu_check": kind}
     return base, acc
@@ -379,7 +396,7 @@ def precision_workload(args):
         err = np.linalg.norm(xc.reshape(k, n) - xf[:k], axis=1) / np.linalg.norm(xf[:k], axis=1)
         out["cpu_baseline"] = {"value": round(k / el, 2), "unit": "systems/s", "cores": O.max_threads(),
                                "kind": kind, "sample": "%d of the %d systems, one PTPR per thread" % (k, nsys),
-                               "rel_l2_vs_fp64_thomas_max": float(err.max())}
+                               "rel_l2_vs_fp64_thomas_max": float(err.max()), "host": host_info()}
     for v in variants:
         v.pop("x", None)
     print(json.dumps(out))
@@ -515,7 +532,8 @@ def uneven_workload(args):
                                "kind": kind,
                                "sample": "%d systems (16 per size class per pass) in %.1f s, one %s per thread; "
                                          "systems/s = equations/s over the batch's mean N" %
-                                         (done, el, args.method.upper())}
+                                         (done, el, args.method.upper()),
+                               "host": host_info()}
     print(json.dumps(out))
 
 
