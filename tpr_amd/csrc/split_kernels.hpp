@@ -141,7 +141,10 @@ __global__ __launch_bounds__(1024) void reduced_kernel(Loc loc) {
 template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
 __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long long nslices) {
     const int l = threadIdx.x % L;
-    const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;
+    // workgroups take the slices in the reverse of K1's order: the slices K1
+    // read last are the likeliest still in the device's last-level cache
+    // when a batch is larger than that cache
+    const long long g = (long long)(gridDim.x - 1 - blockIdx.x) * (blockDim.x / L) + threadIdx.x / L;
     const bool valid = g < nslices;
     long long q;
     int j;
