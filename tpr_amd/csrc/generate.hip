@@ -133,21 +133,39 @@ __device__ __forceinline__ void coef_one(int kind, uint64_t key, int i, int n, f
     abc[2 * (size_t)n + i] = ci;
 }
 
-// A flat grid over the concatenated values of every job: each thread finds
-// its job by binary search over the jobs' first indices.
-__global__ void generate_jobs_kernel(const GenJob* __restrict__ jobs, int njobs, long long total,
-                                     float* __restrict__ base) {
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (long long)gridDim.x * blockDim.x) {
-        int lo = 0, hi = njobs - 1;  // the last job with first <= g
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].first <= g) lo = mid;
-            else hi = mid - 1;
+// A flat grid over the concatenated values of every job, kGenPerThread
+// values per thread at a stride of the block size (coalesced stores).  The
+// block finds the job of its first value by binary search over the jobs'
+// first indices; every thread then walks forward through the jobs as its
+// values advance (a block's 2048 values span a few jobs at most), keeping
+// the job's fields and key in registers.
+constexpr int kGenBlock = 256;
+constexpr int kGenPerThread = 8;
+
+__global__ __launch_bounds__(kGenBlock) void generate_jobs_kernel(const GenJob* __restrict__ jobs, int njobs,
+                                                                 long long total, float* __restrict__ base) {
+    const long long start = (long long)blockIdx.x * (kGenBlock * kGenPerThread);
+    int lo = 0, hi = njobs - 1;  // the last job with first <= start
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= start) lo = mid;
+        else hi = mid - 1;
+    }
+    int j = lo;
+    GenJob jb = jobs[j];
+    uint64_t key = mix64(jb.seed + kGolden * (jb.sys + 1));
+    long long next = j + 1 < njobs ? jobs[j + 1].first : total;
+#pragma unroll
+    for (int i = 0; i < kGenPerThread; i++) {
+        const long long g = start + (long long)i * kGenBlock + threadIdx.x;
+        if (g >= total) break;
+        if (g >= next) {
+            while (j + 1 < njobs && jobs[j + 1].first <= g) j++;
+            jb = jobs[j];
+            key = mix64(jb.seed + kGolden * (jb.sys + 1));
+            next = j + 1 < njobs ? jobs[j + 1].first : total;
         }
-        const GenJob& jb = jobs[lo];
         const uint64_t e = (uint64_t)(g - jb.first);  // value index inside the job
-        const uint64_t key = mix64(jb.seed + kGolden * (jb.sys + 1));
         float* dst = base + jb.dst;
         if (jb.coef_kind == 0) {
             const uint64_t jj = (uint64_t)jb.j0 + (e >> jb.lg_n);
@@ -161,9 +179,11 @@ __global__ void generate_jobs_kernel(const GenJob* __restrict__ jobs, int njobs,
 
 hipError_t launch_generate_jobs(const GenJob* jobs, int njobs, long long total, float* base, hipStream_t stream) {
     if (njobs == 0 || total == 0) return hipSuccess;
-    long long blocks = (total + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(generate_jobs_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, jobs, njobs, total, base);
+    const long long per_block = (long long)kGenBlock * kGenPerThread;
+    const long long blocks = (total + per_block - 1) / per_block;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(generate_jobs_kernel, dim3((unsigned)blocks), dim3(kGenBlock), 0, stream, jobs, njobs, total,
+                       base);
     return hipGetLastError();
 }
 

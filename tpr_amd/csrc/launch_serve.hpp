@@ -21,7 +21,8 @@ struct ServeSys {
 struct ServeLaunch {
     const UnevenTask* tasks;  // work items (the shape ids of launch_uneven.hpp)
     int ntasks;
-    int* counter;  // zeroed by the launch itself
+    int* counter;  // work counter: must hold 0 when the launch starts (the
+                   // serving loop's table copy writes it, no launch of its own)
     const ServeSys* sys;
     float* base;  // arena base
     int grid;

@@ -14,8 +14,6 @@
 
 namespace tprh {
 
-__global__ void serve_zero_kernel(int* counter) { *counter = 0; }
-
 template <int METHOD, int MODE>
 __global__ __launch_bounds__(1024) void serve_kernel(const UnevenTask* __restrict__ tasks, int ntasks,
                                                      int* __restrict__ counter, const ServeSys* __restrict__ sys,
@@ -54,7 +52,6 @@ namespace {
 
 template <int METHOD, int MODE>
 hipError_t launch_serve_t(const ServeLaunch& p, hipStream_t stream) {
-    hipLaunchKernelGGL(serve_zero_kernel, dim3(1), dim3(1), 0, stream, p.counter);
     hipLaunchKernelGGL((serve_kernel<METHOD, MODE>), dim3(p.grid), dim3(1024), p.lds_bytes, stream, p.tasks, p.ntasks,
                        p.counter, p.sys, p.base);
     return hipGetLastError();

@@ -148,7 +148,7 @@ struct Slot {
     hipEvent_t gen_done = nullptr;     // after the batch's generation launch
     std::vector<long long> gen_mats;   // matrices this batch generates
     void* host_stage = nullptr;  // pinned: the batch's packed tables
-    void* dev_table = nullptr;   // device: the same tables, then the work counter
+    void* dev_table = nullptr;   // device: the same tables
     bool busy = false;
     std::vector<long long> reqs;
     long long systems = 0, equations = 0;
@@ -181,7 +181,7 @@ struct tprh_server {
     hipStream_t upload = nullptr;
     int grid = 1;
     size_t lds_bytes = 0;
-    size_t table_bytes = 0;  // capacity of a slot's packed tables; the counter follows
+    size_t table_bytes = 0;  // capacity of a slot's packed tables (the work counter included)
     std::thread worker;
     bool stop = false;
     std::string worker_error;
@@ -204,7 +204,7 @@ void free_matrix_locked(tprh_server* sv, long long id) {
     if (it == sv->mats.end()) return;
     Matrix& m = it->second;
     if (m.ready) {
-        (void)hipEventSynchronize(m.ready);  // its generation or upload is finished before the block is reused
+        (void)hipEventSynchronize(m.ready);  // its upload is finished before the block is reused
         (void)hipEventDestroy(m.ready);
     }
     sv->arena.release(m.off, m.bytes);
@@ -413,6 +413,8 @@ hipError_t launch_batch(tprh_server* sv, Slot& sl, const std::vector<Staged>& it
     for (auto& kv : split_groups)
         split_at.push_back({put(kv.second.data(), sizeof(tprh::ServeSplitSys) * kv.second.size()), kv.second.size()});
     const size_t o_jobs = put(jobs.data(), sizeof(tprh::GenJob) * jobs.size());
+    const int zero = 0;  // the serve kernel's work counter, cleared by this copy
+    const size_t o_counter = tasks.empty() ? 0 : put(&zero, sizeof(int));
     if (at > 0 && (e = hipMemcpyAsync(dt, hs, at, hipMemcpyHostToDevice, sl.stream)) != hipSuccess) return e;
     if (!jobs.empty() &&
         (e = tprh::launch_generate_jobs(reinterpret_cast<const tprh::GenJob*>(dt + o_jobs), (int)jobs.size(),
@@ -424,7 +426,7 @@ hipError_t launch_batch(tprh_server* sv, Slot& sl, const std::vector<Staged>& it
         tprh::ServeLaunch p;
         p.tasks = reinterpret_cast<const tprh::UnevenTask*>(dt + o_tasks);
         p.ntasks = (int)tasks.size();
-        p.counter = reinterpret_cast<int*>(dt + sv->table_bytes);
+        p.counter = reinterpret_cast<int*>(dt + o_counter);
         p.sys = reinterpret_cast<const tprh::ServeSys*>(dt + o_entries);
         p.base = basef;
         p.grid = std::max(1, std::min(sv->grid, p.ntasks));
@@ -669,11 +671,12 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
     // serve kernel's system entries, the split launches' system entries
     // (a batch has at most max_batch_systems right-hand sides, hence at most
     // that many items, entries and requests; split classes add one alignment
-    // pad each, at most 32 of them)
+    // pad each, at most 32 of them), the generation jobs (two per request at
+    // most) and the work counter
     sv->table_bytes = align_up(sizeof(tprh::UnevenTask) * (size_t)cfg->max_batch_systems) +
                       align_up(sizeof(tprh::ServeSys) * (size_t)cfg->max_batch_systems) +
                       align_up(sizeof(tprh::ServeSplitSys) * (size_t)cfg->max_batch_systems) + 32 * kAlign +
-                      align_up(sizeof(tprh::GenJob) * 2 * (size_t)cfg->max_batch_systems);
+                      align_up(sizeof(tprh::GenJob) * 2 * (size_t)cfg->max_batch_systems) + kAlign;
     sv->slots.resize(cfg->max_inflight);
     for (Slot& sl : sv->slots) {
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
@@ -682,7 +685,7 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
             (e = hipEventCreateWithFlags(&sl.gen_done, hipEventDisableTiming)) != hipSuccess)
             return bail(e, "hipEventCreate");
         if ((e = hipHostMalloc(&sl.host_stage, sv->table_bytes)) != hipSuccess) return bail(e, "hipHostMalloc");
-        if ((e = hipMalloc(&sl.dev_table, sv->table_bytes + kAlign)) != hipSuccess) return bail(e, "hipMalloc");
+        if ((e = hipMalloc(&sl.dev_table, sv->table_bytes)) != hipSuccess) return bail(e, "hipMalloc");
         // one table copy of each size class, now: the runtime sets up its
         // copy paths on first use (the first copy of a few tens of KB took
         // about 6 ms of host time on MI355X), which would otherwise stall
