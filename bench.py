@@ -545,7 +545,8 @@ def serve_workload(args):
     N = 2^U{9..15} equations (diagonally dominant generator, generated on the
     device and resident in HBM) and 1..64 right-hand sides d ~ U(-1,1) (ADI-
     like reuse of the matrix); the server admits them against a 64 GB budget
-    and batches continuously.  Reported: systems/s (right-hand sides solved
+    and batches continuously (batching window 1024 right-hand sides / 100 us
+    while other batches run).  Reported: systems/s (right-hand sides solved
     per second), p50/p99 latency from submit to completion; the same request
     mix closed-loop with 256 outstanding (saturated throughput); the plain
     per-system GPU solve (one launch per right-hand side, the reference's
@@ -558,7 +559,12 @@ def serve_workload(args):
     exe = os.path.join(ROOT, "tpr_amd", "tpr_serve_bench")
     if not os.path.exists(exe):
         raise RuntimeError("tpr_amd/tpr_serve_bench is not built: run `python -m tpr_amd.build`")
-    common = ["--method", args.method, "--arith", args.arith, "--seed", "4044", "--budget-gb", "64"]
+    # batching window: while batches run, the next one waits for 1024
+    # right-hand sides or 100 us (fewer, larger batches under load; no effect
+    # at the Poisson rate, where a request usually finds nothing in flight)
+    window = {"min_batch_systems": 1024, "max_wait_us": 100.0}
+    common = ["--method", args.method, "--arith", args.arith, "--seed", "4044", "--budget-gb", "64",
+              "--min-batch", str(window["min_batch_systems"]), "--max-wait-us", str(window["max_wait_us"])]
 
     def client(extra):
         p = subprocess.run([exe] + common + extra, capture_output=True, text=True, timeout=600)
@@ -617,7 +623,7 @@ def serve_workload(args):
         "config": {"workload": "serving loop: Poisson arrivals at %.0f req/s for %.1f s (%d requests), N = 2^U{9..15}, "
                                "1..64 right-hand sides per matrix, matrices generated and resident in HBM, "
                                "64 GB budget, C++ client" % (args.rate, args.seconds, poisson["requests"]),
-                   "method": args.method, "arith": args.arith, "max_inflight": 4},
+                   "method": args.method, "arith": args.arith, "max_inflight": 4, "batching_window": window},
         "latency_ms": poisson["latency_ms"],
         # latency-bound at the offered load; the solve traffic (12 B per matrix
         # equation, 8 B per right-hand-side equation) against the HBM peak
