@@ -87,6 +87,21 @@ def test_serve_bench_client_closed_loop():
     assert d["peak_bytes"] <= 4 << 30
 
 
+@pytest.mark.gpu
+def test_serve_bench_client_threads_and_window():
+    """The load generator with four client threads and a batching window:
+    every request completes (the collector checks each one's state)."""
+    import json
+    exe = os.path.join(ROOT, "tpr_amd", "tpr_serve_bench")
+    p = subprocess.run([exe, "--rate", "0", "--requests", "400", "--outstanding", "64", "--clients", "4",
+                        "--budget-gb", "4", "--min-batch", "256", "--max-wait-us", "200", "--seed", "8"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    d = json.loads(p.stdout.strip().splitlines()[-1])
+    assert d["requests"] == 400 and d["clients"] == 4 and d["min_batch"] == 256
+    assert 0 < d["batches"] <= 400 and d["queued_ms"]["p50"] <= d["latency_ms"]["p99"]
+
+
 def _build_known_answer(tmp_path):
     exe = tmp_path / "known_answer"
     lib = os.path.join(ROOT, "tpr_amd")

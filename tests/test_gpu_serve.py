@@ -222,6 +222,42 @@ def test_serve_wait_any():
         srv.matrix_release(m)
 
 
+def test_serve_concurrent_clients():
+    """Four client threads create matrices, submit, wait and release at the
+    same time (the library calls release the interpreter lock): every answer
+    bit-identical to the oracle, nothing left in the arena."""
+    import threading
+    say("serve concurrent clients")
+    errors = []
+    with Server(256 << 20, arith="exact", max_inflight=4, max_batch_systems=64) as srv:
+        def client(c):
+            try:
+                rng = np.random.default_rng(900 + c)
+                for q in range(8):
+                    n, nrhs = int(1 << rng.integers(9, 14)), int(rng.integers(1, 5))
+                    sid = 5000 + 100 * c + q
+                    m = srv.matrix_generate(n, seed=SEED, sys=sid)
+                    rid = srv.submit(m, nrhs, rhs_seed=SEED, j0=q)
+                    srv.matrix_release(m)  # freed when the request retires
+                    srv.wait(rid)
+                    ref = oracle_answers("ptpr", sid, n, nrhs, j0=q)
+                    if not bits_equal(srv.result(rid).cpu().numpy(), ref):
+                        errors.append((c, q, n, nrhs))
+                    srv.release(rid)
+            except Exception as e:  # reported below, with the thread's id
+                errors.append((c, repr(e)))
+
+        threads = [threading.Thread(target=client, args=(c,)) for c in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in threads)
+        st = srv.stats()
+    assert not errors, errors
+    assert st["completed"] == 32 and st["failed"] == 0 and st["bytes_in_use"] == 0
+
+
 def test_serve_errors():
     with Server(16 << 20) as srv:
         with pytest.raises(_lib.TprError):
