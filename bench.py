@@ -54,11 +54,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
-    ap.add_argument("--workload", default="batch", choices=["batch", "uneven", "precision", "serve", "copy"],
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "uneven", "precision", "serve", "copy", "config0"],
                     help="batch: 4096 x N=1024, S=256 (default, the headline); uneven: 20000 systems of N = 2^U{8..14} "
                          "in one persistent launch; precision: 256 weakly dominant x N=65536, fp16 storage and "
                          "refinement variants; serve: the serving loop under Poisson load; copy: a 256 MiB "
-                         "device copy, the practical HBM ceiling and the counters' calibration")
+                         "device copy, the practical HBM ceiling and the counters' calibration; config0: the "
+                         "reference's documented run, one system N=2048, S=512, 1000 solves")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve workload: mean request arrivals per second")
     ap.add_argument("--seconds", type=float, default=5.0, help="serve workload: length of the Poisson run")
     ap.add_argument("--retune", action="store_true",
@@ -656,6 +658,109 @@ def serve_workload(args):
     print(json.dumps(out))
 
 
+def config0_workload(args):
+    """The reference's documented invocation `tpr_pm 2048 512 1000 PTPR`
+    (README.md:28-30 of the reference): one fp32 system N=2048, S=512 from
+    the diagonally dominant generator (fixed seed), 1000 timed solves.  GPU,
+    two ways: the device-resident solve (tprh_solve_batched on one system,
+    HIP events around 1000 launches replayed from a captured HIP graph, and
+    the same launches issued one by one) and the host-pointer entry point
+    tprh_ptpr_cu (the contract of the reference's ptpr_cu: host arrays with
+    the diagonal divided out, synchronous; wall time per call, PCIe
+    included).  CPU baseline: the reference's PTPR class, 1000 solves one
+    after another with its OpenMP over the m = 4 slices (src/pm.cpp:184-191;
+    the input copy each solve needs is inside the timing, where the
+    reference regenerates it untimed).  rel-L2 against the generator's truth
+    for each."""
+    import numpy as np
+    import torch
+
+    import tpr_amd
+    from tpr_amd import batched as B
+
+    n, s, iters = 2048, 512, 1000
+    torch.cuda.set_device(0)
+    t = B.alloc_batch(1, n)
+    B.generate("dominant", 2048, 0, 1, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
+    torch.cuda.synchronize()
+    def solve():
+        B.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method=args.method, arith=args.arith)
+
+    for _ in range(20):
+        solve()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(run):
+        torch.cuda.synchronize()
+        e0.record()
+        run()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / iters
+
+    # launched one by one from Python, and the same 1000 launches replayed
+    # from a captured HIP graph (no host launch latency between solves)
+    eager_ms = timed(lambda: [solve() for _ in range(iters)])
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(iters):
+            solve()
+    graph.replay()
+    dev_ms = timed(graph.replay) if not args.eager else eager_ms
+    xt = t["xt"].cpu().numpy().astype(np.float64)
+    host = {k: t[k].cpu().numpy() for k in "abcd"}
+
+    def rel(x):
+        return float(np.linalg.norm(x.astype(np.float64) - xt) / np.linalg.norm(xt))
+
+    err_dev = rel(t["x"].cpu().numpy())
+    # the host-pointer path takes the diagonal divided out, as the
+    # reference's 4-argument setter leaves the arrays (fp32 divisions)
+    na, nc, nd = (host[k] / host["b"] for k in "acd")
+    x = np.zeros(n, np.float32)
+    cu = tpr_amd.ptpr_cu if args.method == "ptpr" else tpr_amd.tpr_cu
+    for _ in range(20):
+        cu(na, nc, nd, x, n, s)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        cu(na, nc, nd, x, n, s)
+    cu_ms = (time.perf_counter() - t0) * 1e3 / iters
+    err_cu = rel(x)
+    out = {
+        "metric": METRIC, "value": round(1e3 / dev_ms, 1), "unit": "systems/s", "n_gpus": 1,
+        "steps": iters, "warmup": 20, "ms_per_step": round(dev_ms, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "the reference's documented run: one fp32 system N=2048, S=512, diagonally "
+                               "dominant generator (seed 2048), %d solves" % iters,
+                   "n": n, "s": s, "method": args.method, "arith": args.arith, "graph": not args.eager},
+        "eager_ms_per_solve": round(eager_ms, 5),
+        # one system moves 40 KB: the solve is latency-bound, far below any
+        # bandwidth roof
+        "roofline": {"bound": "hbm", "achieved": round(20 * n / (dev_ms / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(20 * n / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 6),
+                     "traffic": None},
+        "host_pointer_call": {"entry": "tprh_%s_cu" % args.method, "ms_per_call": round(cu_ms, 4),
+                              "rel_l2_vs_truth": err_cu},
+        "accuracy": {"rel_l2_vs_truth": err_dev},
+    }
+    if not args.no_cpu:
+        import oracle as O
+        kind = "reference" if O.ref_available() else "port"
+        solvef = O.ref_solve_batch if kind == "reference" else O.solve_batch
+        reps = {k: np.tile(host[k], iters) for k in "abcd"}
+        solvef(args.method, host["a"], host["b"], host["c"], host["d"], n, s, 0)  # warm
+        t0 = time.perf_counter()
+        xc = solvef(args.method, reps["a"], reps["b"], reps["c"], reps["d"], n, s, 0)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(iters / el, 1), "unit": "systems/s", "cores": O.max_threads(),
+                               "kind": kind, "ms_per_solve": round(el * 1e3 / iters, 4),
+                               "sample": "the same system solved %d times one after another (OpenMP over the "
+                                         "m = %d slices of each solve)" % (iters, n // s),
+                               "rel_l2_vs_truth": rel(xc[:n]), "host": host_info()}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -673,6 +778,9 @@ def main():
         return
     if args.workload == "copy":
         copy_workload(args)
+        return
+    if args.workload == "config0":
+        config0_workload(args)
         return
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps
