@@ -294,13 +294,14 @@ struct Staged {
 };
 
 // Builds and launches one batch on slot `sl` (busy, so this thread owns its
-// stream and tables) from the staged requests; called without the lock.  Every table of the batch (work
-// items, serve-kernel entries, split entries, right-hand-side jobs) is
+// stream and tables) from the staged requests; called without the lock.
+// Every table of the batch (work items, serve-kernel entries, split
+// entries, generation jobs, the serve kernel's zeroed work counter) is
 // packed into the slot's pinned staging buffer and copied with one
 // transfer; then, on the slot's stream: caller right-hand sides are copied
-// in, synthetic ones and the coefficients of matrices first used here
-// generated (one launch, then the gen_done event that batches on other slots
-// using those matrices wait for), the serve kernel runs for every
+// in, synthetic ones and the coefficients of matrices first used here are
+// generated (one launch, then the gen_done event that batches on other
+// slots using those matrices wait for), the serve kernel runs for every
 // class within a workgroup, one split launch set per larger class, and the
 // answers the caller asked for are copied out.  Returns the first HIP error.
 hipError_t launch_batch(tprh_server* sv, Slot& sl, const std::vector<Staged>& items) {
