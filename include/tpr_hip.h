@@ -121,7 +121,8 @@ tprh_status tprh_solver_destroy(tprh_solver *h);
  * unit diagonal.  Asynchronous on `stream` (a hipStream_t; NULL = the null
  * stream); no host synchronisation, no allocation, graph-capturable.
  * method: TPRH_PTPR or TPRH_TPR.  The slice shape is the default for s
- * (s = 512: 8 equations per lane x 64 lanes while n / s <= 16). */
+ * (s = 512: 8 equations per lane x 64 lanes while n / s <= 8 for PTPR,
+ * <= 4 for TPR; 16 x 32 otherwise). */
 tprh_status tprh_solve_batched(tprh_method method, tprh_arith arith, const float *a, const float *b,
                                const float *c, const float *d, float *x, int n, int s, int nsys,
                                void *stream);

@@ -42,14 +42,24 @@ tprh_status fail_hip(hipError_t e, const char* what) {
 
 bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// Largest slices-per-system for which s = 512 takes the 8 x 64 shape, as
+// measured on MI355X (fast arithmetic, 1 to 2048 systems): PTPR gains
+// 12-37 % up to m = 8 and loses ~3 % on large batches at m = 16 (a
+// 1024-lane workgroup per system); TPR gains 3-6 % at m = 4, is within
+// +-3 % at m = 8 and loses up to 34 % at m = 16.  Results are bit-identical
+// between the two shapes in both arithmetic modes.
+int wide_max_of(int method) { return method == TPRH_PTPR ? 8 : 4; }
+
 // Checks (n, s) for PTPR/TPR and picks the slice shape.  The reference
 // requires s and m = n / s to be powers of two (src/ptpr.cpp:51-52,
 // src/pcr.cpp:16) and silently misbehaves otherwise; here they are errors.
 // *split is set when a system needs more lanes than one workgroup has.
-// wide_ok: the caller's variant exists for every compiled shape (fp32
-// coefficients, no refinement), so the default may pick a wider one.
+// wide_max_m: for s = 512 the default takes 8 equations per lane x 64
+// lanes instead of 16 x 32 when the system has at most this many slices
+// (0: never; the caller's variant must exist for every compiled shape, i.e.
+// fp32 coefficients and no refinement).
 tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape, bool allow_split, bool* split,
-                     bool wide_ok = false) {
+                     int wide_max_m = 0) {
     if (n <= 0 || s <= 0 || nsys < 0) return fail(TPRH_ERROR_INVALID_VALUE, "n, s must be positive and nsys >= 0");
     if (!is_pow2(s) || s < 2) return fail(TPRH_ERROR_INVALID_VALUE, "s must be a power of two >= 2");
     if (n % s != 0) return fail(TPRH_ERROR_INVALID_VALUE, "n must be a multiple of s");
@@ -60,10 +70,9 @@ tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape
         if (!tprh::shape_compiled(*E, *L)) return fail(TPRH_ERROR_NOT_SUPPORTED, "slice shape not compiled in");
     } else if (!tprh::default_shape(s, E, L)) {
         return fail(TPRH_ERROR_NOT_SUPPORTED, "slice size s above 2048 is not supported");
-    } else if (wide_ok && s == 512 && (long long)m * 64 <= 1024) {
+    } else if (s == 512 && m <= wide_max_m) {
         // half the equations per lane of (16, 32): shorter dependent chains
-        // per lane, 1.3-1.6x faster on MI355X from one system to 4096, while
-        // the system still fits one workgroup (m * 64 lanes <= 1024)
+        // per lane (see wide_max_of for where it pays)
         *E = 8;
         *L = 64;
     }
@@ -108,7 +117,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     pl->E = d.E;
     pl->L = d.L;
     tprh_status st = check_ns(d.n, d.s, d.nsys, &pl->E, &pl->L, explicit_shape, true, &pl->split,
-                              d.coef == TPRH_STORE_F32 && !d.refine);
+                              d.coef == TPRH_STORE_F32 && !d.refine ? wide_max_of(d.method) : 0);
     if (st != TPRH_SUCCESS) return st;
     if ((d.coef == TPRH_STORE_F16 || d.refine) && !tprh::is_default_shape(pl->E, pl->L))
         return fail(TPRH_ERROR_NOT_SUPPORTED, "fp16 storage and refinement use the default slice shapes only");
@@ -163,7 +172,7 @@ tprh_status batched(tprh_method method, tprh_arith arith, const float* a, const 
     if (method != TPRH_PTPR && method != TPRH_TPR)
         return fail(TPRH_ERROR_INVALID_VALUE, "batched solve: method must be TPRH_PTPR or TPRH_TPR");
     bool split = false;
-    tprh_status st = check_ns(n, s, nsys, &E, &L, explicit_shape, false, &split, true);
+    tprh_status st = check_ns(n, s, nsys, &E, &L, explicit_shape, false, &split, wide_max_of(method));
     if (st != TPRH_SUCCESS) return st;
     tprh_solve_desc desc{};
     desc.method = method;
