@@ -191,12 +191,17 @@ tprh_status batched(tprh_method method, tprh_arith arith, const float* a, const 
     return solve_desc(desc, (hipStream_t)stream);
 }
 
-// Device buffers for the host-pointer entry points, one per device, grown on
+// Buffers for the host-pointer entry points, one set per device, grown on
 // demand and kept for the life of the process (the reference allocates and
-// frees five buffers per call, src/ptpr.cu:554-598).
+// frees five buffers per call, src/ptpr.cu:554-598): device arrays
+// a | c | rhs | diag | x | workspace, and a pinned host copy of the first
+// five, through which a call moves its inputs in one transfer and its
+// answer in another.
 struct HostPathBuffer {
     float* dev = nullptr;
     size_t cap = 0;  // floats
+    float* pinned = nullptr;
+    size_t pcap = 0;  // floats
 };
 struct HostPathBuffers {
     std::mutex mu;
@@ -246,17 +251,28 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
         TPRH_HIP(hipMalloc(&buf.dev, need * sizeof(float)));
         buf.cap = need;
     }
+    if (buf.pcap < 5 * (size_t)n) {
+        if (buf.pinned) (void)hipHostFree(buf.pinned);
+        buf.pinned = nullptr;
+        buf.pcap = 0;
+        TPRH_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf.pinned), 5 * (size_t)n * sizeof(float)));
+        buf.pcap = 5 * (size_t)n;
+    }
     float* da = buf.dev;
     float* dc = da + n;
     float* dr = dc + n;
-    float* dx = dr + n;
-    float* db = dx + n;
-    float* ws = db + n;
+    float* db = dr + n;
+    float* dx = db + n;
+    float* ws = dx + n;
+    float* hp = buf.pinned;  // same layout as the device arrays
     const size_t bytes = sizeof(float) * (size_t)n;
-    TPRH_HIP(hipMemcpy(da, a, bytes, hipMemcpyHostToDevice));
-    TPRH_HIP(hipMemcpy(dc, c, bytes, hipMemcpyHostToDevice));
-    TPRH_HIP(hipMemcpy(dr, rhs, bytes, hipMemcpyHostToDevice));
-    if (diag) TPRH_HIP(hipMemcpy(db, diag, bytes, hipMemcpyHostToDevice));
+    // a call that failed after its upload may have left that copy queued
+    TPRH_HIP(hipStreamSynchronize(nullptr));
+    std::memcpy(hp, a, bytes);
+    std::memcpy(hp + n, c, bytes);
+    std::memcpy(hp + 2 * (size_t)n, rhs, bytes);
+    if (diag) std::memcpy(hp + 3 * (size_t)n, diag, bytes);
+    TPRH_HIP(hipMemcpyAsync(da, hp, (diag ? 4 : 3) * bytes, hipMemcpyHostToDevice, nullptr));
     EventPair timing;
     hipEvent_t* ev = timing.ev;
     if (g_timing) {
@@ -288,7 +304,9 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     }
     if (st != TPRH_SUCCESS) return st;
     TPRH_HIP(le);
-    TPRH_HIP(hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost));
+    TPRH_HIP(hipMemcpyAsync(hp + 4 * (size_t)n, dx, bytes, hipMemcpyDeviceToHost, nullptr));
+    TPRH_HIP(hipStreamSynchronize(nullptr));
+    std::memcpy(x, hp + 4 * (size_t)n, bytes);
     return TPRH_SUCCESS;
 }
 
