@@ -123,6 +123,21 @@ def test_alternative_shapes_bit_identical(s, E, L, method):
     assert bits_equal(x, O.solve_batch(method, a, b, c, d, n, s))
 
 
+@pytest.mark.parametrize("n", [1024, 2048, 4096, 8192])
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_s512_default_shape_matches_16x32(method, n):
+    """s = 512 takes 8 x 64 slices where that measured faster (PTPR up to 8
+    slices per system, TPR up to 4) and 16 x 32 otherwise: in fast arithmetic
+    too the default's answers equal the 16 x 32 answers bit for bit, so the
+    choice never changes a result."""
+    s, nsys = 512, 5
+    say("s512 default", method, n)
+    a, b, c, d, _ = O.gen_batch(11, 0, nsys, n)
+    x_def = gpu_solve(a, b, c, d, n, s, method, "fast")
+    x_ref = gpu_solve(a, b, c, d, n, s, method, "fast", shape=(16, 32))
+    assert bits_equal(x_def, x_ref)
+
+
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
 def test_full_batch_4096x1024(method):
     """4096 systems x N = 1024, S = 256, generated on the GPU: exact is
