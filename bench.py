@@ -23,6 +23,12 @@ CPU baseline: the reference's own PTPR (oracle/_ref, compiled from its
 sources; the oracle's restatement where that build is absent), one solver
 per thread over the systems, on this host's cores, on a bounded sample of the
 same batch; the GPU answers of that sample are compared with it (rel-L2).
+
+Side figures in the same line: the other method (TPR beside PTPR), the
+rocSPARSE comparator, a device copy (the practical HBM ceiling) and
+"batch_stream": the same batches issued over 2 and 3 streams so that
+consecutive launches overlap (one batch's compute tail under the next one's
+loads), over eight buffer sets.
 """
 import argparse
 import ctypes
@@ -128,7 +134,74 @@ def gpu_leg(args):
     host = {k: sets[0][k].cpu().numpy() for k in ("a", "b", "c", "d", "xt")}
     vendor = None if args.no_vendor else vendor_leg(sets, host)
     other = None if args.eager else other_method_leg(args, sets, shape)
-    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host, "vendor": vendor, "other": other}
+    streams = None if args.eager else batch_stream_leg(args, sets, shape)
+    return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host, "vendor": vendor, "other": other,
+            "streams": streams}
+
+
+def batch_stream_leg(args, sets, shape):
+    """The same batches as a stream of independent launches spread over 2
+    and 3 streams (default priority), so that one batch's compute tail runs
+    under the next one's loads: the throughput of a caller that keeps
+    several batches in flight.  A side figure, not the headline (each launch
+    still solves one 4096-system batch, but launches overlap, so a kernel
+    trace's per-launch durations are longer than the time per batch here).
+    Eight buffer sets (671 MB, well past the 256 MB last-level cache), K
+    launches captured in one graph with a fork and a join, HIP events
+    around the replay."""
+    import torch
+
+    import tpr_amd
+    from tpr_amd import batched as B
+
+    nsets = 2 * ROT
+    pool = list(sets)
+    for r in range(len(pool), nsets):
+        t = B.alloc_batch(NSYS, N, with_xt=False)
+        B.generate("dominant", 20240 + r, 0, NSYS, N, t["a"], t["b"], t["c"], t["d"], None)
+        pool.append(t)
+    torch.cuda.synchronize()
+
+    def step(i):
+        t = pool[i % nsets]
+        tpr_amd.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith,
+                              shape=shape)
+
+    out = {"buffer_sets": nsets}
+    for ns in (2, 3):
+        side = [torch.cuda.Stream() for _ in range(ns - 1)]
+        for i in range(nsets):
+            step(i)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cap = torch.cuda.current_stream()
+            fork = torch.cuda.Event()
+            fork.record(cap)
+            for s in side:
+                s.wait_event(fork)
+            for i in range(args.steps):
+                if i % ns == 0:
+                    step(i)
+                else:
+                    with torch.cuda.stream(side[i % ns - 1]):
+                        step(i)
+            for s in side:
+                join = torch.cuda.Event()
+                join.record(s)
+                cap.wait_event(join)
+        g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        gbps = 20.0 * N * NSYS / (ms / 1e3) / 1e9
+        out["%d_streams" % ns] = {"ms_per_batch": round(ms, 5), "systems_per_s": round(NSYS / (ms / 1e3), 1),
+                                  "GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def other_method_leg(args, sets, shape):
@@ -822,6 +895,8 @@ def main():
                            "what": "256 MiB device copy measured in this run (read + write bytes / time)"}
     if g.get("other") is not None:
         out["other_method"] = g["other"]
+    if g.get("streams") is not None:
+        out["batch_stream"] = g["streams"]
     if g.get("vendor") is not None:
         out["vendor_comparator"] = g["vendor"]
     if not args.no_cpu:
