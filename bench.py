@@ -893,6 +893,15 @@ def main():
     cg, _ = copy_rate()
     out["copy_ceiling"] = {"GBps": round(cg, 1), "frac": round(achieved / cg, 4),
                            "what": "256 MiB device copy measured in this run (read + write bytes / time)"}
+    # the reference's analytic operation count (src/ptpr.cpp:104-106,
+    # src/tpr.cpp:96-98), as the solver class's solve() returns it
+    import tpr_amd
+    solver = (tpr_amd.PTPR if args.method == "ptpr" else tpr_amd.TPR)(N, S)
+    h = g["host"]
+    solver.set_tridiagonal_system(*(h[k][:N] for k in "abcd"))
+    per_sys = int(solver.solve())
+    out["analytic_flops"] = {"per_system": per_sys, "GFLOPs": round(per_sys * value / 1e9, 1),
+                             "source": "the reference's count returned by solve(), times systems/s"}
     if g.get("other") is not None:
         out["other_method"] = g["other"]
     if g.get("streams") is not None:

@@ -47,3 +47,6 @@ def test_json_line(args):
     if "--workload" not in args:
         assert d["steps"] == 10 and d["warmup"] == 2
         assert d["accuracy"]["rel_l2_vs_cpu_max"] <= 1e-5
+        # the reference's analytic count for PTPR, N = 1024, S = 256
+        assert d["analytic_flops"]["per_system"] == 118963
+        assert set(d["batch_stream"]) == {"buffer_sets", "2_streams", "3_streams"}
