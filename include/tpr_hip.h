@@ -120,9 +120,10 @@ tprh_status tprh_solver_destroy(tprh_solver *h);
  * [k*n, (k+1)*n) of a, b, c, d and x, all device pointers.  b == NULL means a
  * unit diagonal.  Asynchronous on `stream` (a hipStream_t; NULL = the null
  * stream); no host synchronisation, no allocation, graph-capturable.
- * method: TPRH_PTPR or TPRH_TPR.  The slice shape is the default for s
- * (s = 512: 8 equations per lane x 64 lanes while n / s <= 8 for PTPR,
- * <= 4 for TPR; 16 x 32 otherwise). */
+ * method: TPRH_PTPR or TPRH_TPR.  The slice shape is the default for s,
+ * widened where that measured faster without changing a result: s = 512
+ * takes 8 equations per lane x 64 lanes while n / s <= 8 (PTPR) or <= 4
+ * (TPR); exact PTPR at s = 256 takes 4 x 64 while n / s <= 16. */
 tprh_status tprh_solve_batched(tprh_method method, tprh_arith arith, const float *a, const float *b,
                                const float *c, const float *d, float *x, int n, int s, int nsys,
                                void *stream);

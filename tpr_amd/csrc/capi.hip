@@ -42,24 +42,41 @@ tprh_status fail_hip(hipError_t e, const char* what) {
 
 bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 
-// Largest slices-per-system for which s = 512 takes the 8 x 64 shape, as
-// measured on MI355X (fast arithmetic, 1 to 2048 systems): PTPR gains
-// 12-37 % up to m = 8 and loses ~3 % on large batches at m = 16 (a
-// 1024-lane workgroup per system); TPR gains 3-6 % at m = 4, is within
-// +-3 % at m = 8 and loses up to 34 % at m = 16.  Results are bit-identical
-// between the two shapes in both arithmetic modes.
-int wide_max_of(int method) { return method == TPRH_PTPR ? 8 : 4; }
+// Where a wider slice shape than default_shape()'s measured faster on MI355X
+// (1 to 4096 systems, inputs in HBM), with m = n / s slices per system:
+//   s = 512, 8 x 64 instead of 16 x 32: PTPR up to m = 8 (fast 12-37 %,
+//     exact 14-39 %; fast loses ~3 % on large batches at m = 16, a 1024-lane
+//     workgroup per system), TPR up to m = 4 (fast 3-6 %; within +-3 % at
+//     m = 8, up to 34 % slower at m = 16);
+//   s = 256, 4 x 64 instead of 16 x 16: exact PTPR up to m = 16 (13-47 %);
+//     fast PTPR and both TPR modes keep 16 x 16 (fast PTPR, 4096 systems of
+//     1024: 21 us against 28 us).
+// The answers are the same bits with either shape in these cases (exact
+// arithmetic always; fast arithmetic for the s = 512 pair), so the choice
+// changes speed only.
+bool wider_shape(int method, int arith, int s, int m, int* E, int* L) {
+    if (s == 512 && m <= (method == TPRH_PTPR ? 8 : 4)) {
+        *E = 8;
+        *L = 64;
+        return true;
+    }
+    if (s == 256 && method == TPRH_PTPR && arith == TPRH_ARITH_EXACT && m <= 16) {
+        *E = 4;
+        *L = 64;
+        return true;
+    }
+    return false;
+}
 
 // Checks (n, s) for PTPR/TPR and picks the slice shape.  The reference
 // requires s and m = n / s to be powers of two (src/ptpr.cpp:51-52,
 // src/pcr.cpp:16) and silently misbehaves otherwise; here they are errors.
 // *split is set when a system needs more lanes than one workgroup has.
-// wide_max_m: for s = 512 the default takes 8 equations per lane x 64
-// lanes instead of 16 x 32 when the system has at most this many slices
-// (0: never; the caller's variant must exist for every compiled shape, i.e.
-// fp32 coefficients and no refinement).
+// wide_method >= 0: the caller's variant exists for every compiled shape
+// (fp32 coefficients, no refinement), so the default may take the wider
+// shape wider_shape() names for that method and arithmetic.
 tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape, bool allow_split, bool* split,
-                     int wide_max_m = 0) {
+                     int wide_method = -1, int wide_arith = 0) {
     if (n <= 0 || s <= 0 || nsys < 0) return fail(TPRH_ERROR_INVALID_VALUE, "n, s must be positive and nsys >= 0");
     if (!is_pow2(s) || s < 2) return fail(TPRH_ERROR_INVALID_VALUE, "s must be a power of two >= 2");
     if (n % s != 0) return fail(TPRH_ERROR_INVALID_VALUE, "n must be a multiple of s");
@@ -70,11 +87,8 @@ tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape
         if (!tprh::shape_compiled(*E, *L)) return fail(TPRH_ERROR_NOT_SUPPORTED, "slice shape not compiled in");
     } else if (!tprh::default_shape(s, E, L)) {
         return fail(TPRH_ERROR_NOT_SUPPORTED, "slice size s above 2048 is not supported");
-    } else if (s == 512 && m <= wide_max_m) {
-        // half the equations per lane of (16, 32): shorter dependent chains
-        // per lane (see wide_max_of for where it pays)
-        *E = 8;
-        *L = 64;
+    } else if (wide_method >= 0) {
+        (void)wider_shape(wide_method, wide_arith, s, m, E, L);
     }
     *split = (long long)m * *L > 1024;
     if (*split) {
@@ -117,7 +131,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     pl->E = d.E;
     pl->L = d.L;
     tprh_status st = check_ns(d.n, d.s, d.nsys, &pl->E, &pl->L, explicit_shape, true, &pl->split,
-                              d.coef == TPRH_STORE_F32 && !d.refine ? wide_max_of(d.method) : 0);
+                              d.coef == TPRH_STORE_F32 && !d.refine ? (int)d.method : -1, (int)d.arith);
     if (st != TPRH_SUCCESS) return st;
     if ((d.coef == TPRH_STORE_F16 || d.refine) && !tprh::is_default_shape(pl->E, pl->L))
         return fail(TPRH_ERROR_NOT_SUPPORTED, "fp16 storage and refinement use the default slice shapes only");
@@ -172,7 +186,7 @@ tprh_status batched(tprh_method method, tprh_arith arith, const float* a, const 
     if (method != TPRH_PTPR && method != TPRH_TPR)
         return fail(TPRH_ERROR_INVALID_VALUE, "batched solve: method must be TPRH_PTPR or TPRH_TPR");
     bool split = false;
-    tprh_status st = check_ns(n, s, nsys, &E, &L, explicit_shape, false, &split, wide_max_of(method));
+    tprh_status st = check_ns(n, s, nsys, &E, &L, explicit_shape, false, &split, (int)method, (int)arith);
     if (st != TPRH_SUCCESS) return st;
     tprh_solve_desc desc{};
     desc.method = method;
