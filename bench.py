@@ -57,6 +57,9 @@ def parse():
                     help="launch the timed steps one by one from Python instead of replaying a captured HIP graph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator leg")
+    ap.add_argument("--no-streams", action="store_true",
+                    help="skip the batch_stream leg (overlapping launches), e.g. under a kernel trace whose "
+                         "per-launch average should be the single-stream figure")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
@@ -134,7 +137,7 @@ def gpu_leg(args):
     host = {k: sets[0][k].cpu().numpy() for k in ("a", "b", "c", "d", "xt")}
     vendor = None if args.no_vendor else vendor_leg(sets, host)
     other = None if args.eager else other_method_leg(args, sets, shape)
-    streams = None if args.eager else batch_stream_leg(args, sets, shape)
+    streams = None if (args.eager or args.no_streams) else batch_stream_leg(args, sets, shape)
     return {"wall_s": wall, "event_ms": ev_ms, "x0": x0, "host": host, "vendor": vendor, "other": other,
             "streams": streams}
 
