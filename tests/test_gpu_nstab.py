@@ -5,7 +5,6 @@ C++ driver prints the same CSV from the dense-CSV text files; the rocSPARSE
 comparator solves dominant batches to fp32 accuracy."""
 import os
 import subprocess
-import sys
 
 import numpy as np
 import pytest

@@ -2,7 +2,6 @@
 before any device call, and null handles are refused."""
 import ctypes
 
-import numpy as np
 import pytest
 
 from tpr_amd import _lib
