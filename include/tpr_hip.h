@@ -123,7 +123,9 @@ tprh_status tprh_solver_destroy(tprh_solver *h);
  * method: TPRH_PTPR or TPRH_TPR.  The slice shape is the default for s,
  * widened where that measured faster without changing a result: s = 512
  * takes 8 equations per lane x 64 lanes while n / s <= 8 (PTPR) or <= 4
- * (TPR); exact PTPR at s = 256 takes 4 x 64 while n / s <= 16. */
+ * (TPR); exact PTPR at s = 256 takes 4 x 64 while n / s <= 16, exact TPR
+ * at s = 256 takes 8 x 32 in batches of at most 256 systems while
+ * n / s <= 8. */
 tprh_status tprh_solve_batched(tprh_method method, tprh_arith arith, const float *a, const float *b,
                                const float *c, const float *d, float *x, int n, int s, int nsys,
                                void *stream);

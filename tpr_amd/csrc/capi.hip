@@ -49,12 +49,14 @@ bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 //     workgroup per system), TPR up to m = 4 (fast 3-6 %; within +-3 % at
 //     m = 8, up to 34 % slower at m = 16);
 //   s = 256, 4 x 64 instead of 16 x 16: exact PTPR up to m = 16 (13-47 %);
-//     fast PTPR and both TPR modes keep 16 x 16 (fast PTPR, 4096 systems of
+//   s = 256, 8 x 32 instead of 16 x 16: exact TPR up to m = 8 in batches of
+//     at most 256 systems (12-26 %; equal or slower from 1024 systems on);
+//     fast PTPR and fast TPR keep 16 x 16 (fast PTPR, 4096 systems of
 //     1024: 21 us against 28 us).
 // The answers are the same bits with either shape in these cases (exact
 // arithmetic always; fast arithmetic for the s = 512 pair), so the choice
-// changes speed only.
-bool wider_shape(int method, int arith, int s, int m, int* E, int* L) {
+// changes speed only; in exact arithmetic it may depend on the batch size.
+bool wider_shape(int method, int arith, int s, int m, int nsys, int* E, int* L) {
     if (s == 512 && m <= (method == TPRH_PTPR ? 8 : 4)) {
         *E = 8;
         *L = 64;
@@ -63,6 +65,11 @@ bool wider_shape(int method, int arith, int s, int m, int* E, int* L) {
     if (s == 256 && method == TPRH_PTPR && arith == TPRH_ARITH_EXACT && m <= 16) {
         *E = 4;
         *L = 64;
+        return true;
+    }
+    if (s == 256 && method == TPRH_TPR && arith == TPRH_ARITH_EXACT && m <= 8 && nsys <= 256) {
+        *E = 8;
+        *L = 32;
         return true;
     }
     return false;
@@ -88,7 +95,7 @@ tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape
     } else if (!tprh::default_shape(s, E, L)) {
         return fail(TPRH_ERROR_NOT_SUPPORTED, "slice size s above 2048 is not supported");
     } else if (wide_method >= 0) {
-        (void)wider_shape(wide_method, wide_arith, s, m, E, L);
+        (void)wider_shape(wide_method, wide_arith, s, m, nsys, E, L);
     }
     *split = (long long)m * *L > 1024;
     if (*split) {
