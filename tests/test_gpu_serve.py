@@ -258,6 +258,23 @@ def test_serve_concurrent_clients():
     assert st["completed"] == 32 and st["failed"] == 0 and st["bytes_in_use"] == 0
 
 
+def test_serve_destroy_drains():
+    """Destroying the server with requests queued and in flight waits for all
+    of them: every caller buffer then holds its answers, bit-identical to the
+    oracle."""
+    n, nrhs, nreq = 4096, 8, 12
+    say("serve destroy drains")
+    srv = Server(64 << 20, arith="exact", max_inflight=2, max_batch_systems=nrhs)
+    m = srv.matrix_generate(n, seed=SEED, sys=4300)
+    xs = [torch.full((nrhs * n,), float("nan"), device="cuda") for _ in range(nreq)]
+    for q in range(nreq):
+        srv.submit(m, nrhs, x=xs[q], rhs_seed=SEED, j0=nrhs * q)
+    srv.close()
+    for q in range(nreq):
+        ref = oracle_answers("ptpr", 4300, n, nrhs, j0=nrhs * q)
+        assert bits_equal(xs[q].cpu().numpy().reshape(nrhs, n), ref), q
+
+
 def test_serve_errors():
     with Server(16 << 20) as srv:
         with pytest.raises(_lib.TprError):
