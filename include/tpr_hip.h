@@ -137,6 +137,10 @@ tprh_status tprh_solve_batched_shape(tprh_method method, tprh_arith arith, const
                                      const float *c, const float *d, float *x, int n, int s, int nsys,
                                      int E, int L, void *stream);
 int tprh_shape_supported(int E, int L);
+/* The slice shape (E equations per lane, L lanes per slice) that
+ * tprh_solve_batched takes for this method, arithmetic, n, s and batch size;
+ * no device call.  Errors as tprh_solve_batched's for (n, s). */
+tprh_status tprh_slice_shape(tprh_method method, tprh_arith arith, int n, int s, int nsys, int *E, int *L);
 
 /* ---- the general entry point ---------------------------------------------
  * Everything tprh_solve_batched does, plus:

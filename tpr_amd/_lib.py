@@ -45,6 +45,7 @@ SIGNATURES = {
     "tprh_solve_batched": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "tprh_solve_batched_shape": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "tprh_shape_supported": (_i, [_i, _i]),
+    "tprh_slice_shape": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "tprh_generate": (_i, [_i, _u64, _u64, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 

@@ -123,6 +123,15 @@ def shape_supported(E, L):
     return bool(_lib.lib().tprh_shape_supported(int(E), int(L)))
 
 
+def slice_shape(n, s, nsys=1, method="ptpr", arith="fast"):
+    """(E, L): the slice shape solve_batched takes for these arguments (no
+    device call)."""
+    E, L = ctypes.c_int(), ctypes.c_int()
+    _lib.call("tprh_slice_shape", METHODS[method], ARITH[arith], int(n), int(s), int(nsys), ctypes.byref(E),
+              ctypes.byref(L))
+    return E.value, L.value
+
+
 def generate(kind, seed, sys0, nsys, n, a, b, c, d, xt=None, stream=None):
     """Fill device tensors with the seeded synthetic systems sys0 .. sys0+nsys-1."""
     tot = nsys * n

@@ -540,6 +540,19 @@ tprh_status tprh_solve_batched_shape(tprh_method method, tprh_arith arith, const
 
 int tprh_shape_supported(int E, int L) { return tprh::shape_compiled(E, L) ? 1 : 0; }
 
+tprh_status tprh_slice_shape(tprh_method method, tprh_arith arith, int n, int s, int nsys, int* E, int* L) {
+    if (!E || !L) return fail(TPRH_ERROR_INVALID_VALUE, "null argument");
+    if (method != TPRH_PTPR && method != TPRH_TPR) return fail(TPRH_ERROR_INVALID_VALUE, "method must be PTPR or TPR");
+    if (arith != TPRH_ARITH_FAST && arith != TPRH_ARITH_EXACT) return fail(TPRH_ERROR_INVALID_VALUE, "bad arith");
+    bool split = false;
+    int e = 0, l = 0;
+    tprh_status st = check_ns(n, s, nsys, &e, &l, false, false, &split, (int)method, (int)arith);
+    if (st != TPRH_SUCCESS) return st;
+    *E = e;
+    *L = l;
+    return TPRH_SUCCESS;
+}
+
 size_t tprh_workspace_bytes(const tprh_solve_desc* desc) {
     if (!desc) return (size_t)-1;
     Plan pl;
