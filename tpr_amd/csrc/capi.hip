@@ -352,7 +352,7 @@ tprh_status set_hip_error(hipError_t e, const char* what) { return fail_hip(e, w
 // (NULL or a zero row: s = min(n, 256), default shape).  *split: the class
 // needs the multi-workgroup path.
 tprh_status class_shape(int n, const int* schedule, int* s_out, int* E_out, int* L_out, bool allow_split,
-                        bool* split) {
+                        bool* split, int method, int arith) {
     if (n < 16 || !is_pow2(n)) return fail(TPRH_ERROR_INVALID_VALUE, "system sizes must be powers of two >= 16");
     const int lg = __builtin_ctz((unsigned)n);
     int s = n < 256 ? n : 256, E = 0, L = 0;
@@ -363,7 +363,11 @@ tprh_status class_shape(int n, const int* schedule, int* s_out, int* E_out, int*
         L = schedule[3 * lg + 2];
         expl = E != 0 || L != 0;
     }
-    tprh_status st = check_ns(n, s, 1, &E, &L, expl, allow_split, split);
+    // exact arithmetic gives the same bits with any shape, so a class without
+    // a schedule row may take the wider shapes the batched entry points take
+    // (the persistent kernels carry them: 8 x 32, 4 x 64, 8 x 64)
+    tprh_status st = check_ns(n, s, 1, &E, &L, expl, allow_split, split,
+                              arith == TPRH_ARITH_EXACT ? method : -1, arith);
     if (st != TPRH_SUCCESS) return st;
     *s_out = s;
     *E_out = E;
@@ -586,7 +590,7 @@ tprh_status tprh_uneven_plan_create(tprh_uneven_plan** out, tprh_method method, 
         if (!classes.count(nk)) {
             int s = 0, E = 0, L = 0;
             bool split = false;
-            tprh_status st = tprh::class_shape(nk, schedule, &s, &E, &L, false, &split);
+            tprh_status st = tprh::class_shape(nk, schedule, &s, &E, &L, false, &split, (int)method, (int)arith);
             if (st != TPRH_SUCCESS) return st;
             if (tprh::uneven_shape_id(E, L) < 0)
                 return fail(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the uneven-batch kernel");

@@ -46,7 +46,8 @@
 namespace tprh {
 tprh_status set_error(tprh_status st, const std::string& msg);
 tprh_status set_hip_error(hipError_t e, const char* what);
-tprh_status class_shape(int n, const int* schedule, int* s, int* E, int* L, bool allow_split, bool* split);
+tprh_status class_shape(int n, const int* schedule, int* s, int* E, int* L, bool allow_split, bool* split,
+                        int method, int arith);
 }  // namespace tprh
 
 namespace {
@@ -595,7 +596,8 @@ void worker_main(tprh_server* sv) {
 }
 
 tprh_status validate_n(tprh_server* sv, int n, int* s, int* E, int* L, bool* split) {
-    tprh_status st = tprh::class_shape(n, sv->have_schedule ? sv->schedule : nullptr, s, E, L, true, split);
+    tprh_status st = tprh::class_shape(n, sv->have_schedule ? sv->schedule : nullptr, s, E, L, true, split,
+                                       (int)sv->cfg.method, (int)sv->cfg.arith);
     if (st != TPRH_SUCCESS) return st;
     if (*split && !tprh::serve_split_shape(*E, *L))
         return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "size classes above one workgroup need s >= 256 when served");
@@ -628,7 +630,8 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
             if (cfg->schedule[3 * lg] == 0) continue;
             int s, E, L;
             bool split;
-            tprh_status st = tprh::class_shape(1 << lg, cfg->schedule, &s, &E, &L, true, &split);
+            tprh_status st =
+                tprh::class_shape(1 << lg, cfg->schedule, &s, &E, &L, true, &split, (int)cfg->method, (int)cfg->arith);
             if (st != TPRH_SUCCESS) return st;
             if (!split && tprh::uneven_shape_id(E, L) < 0)
                 return tprh::set_error(TPRH_ERROR_NOT_SUPPORTED, "slice shape has no path in the serve kernel");
