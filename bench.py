@@ -574,6 +574,12 @@ def uneven_workload(args):
                      "frac": best["hbm_frac"], "traffic": None},
         "schedules": res,
     }
+    # HBM bytes per persistent launch from the FETCH_SIZE / WRITE_SIZE runs of
+    # this workload, scaled by the copy calibration (fast arithmetic)
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_uneven20000.json" % args.method)
+    if args.arith == "fast" and os.path.exists(tf):
+        with open(tf) as f:
+            out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
     if not args.no_cpu:
         import oracle as O
         pa = plans["autotuned"]
