@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="precision workload: the fp32 solve alone (no fp16, refinement or per-system variants), "
+                         "e.g. for counter runs")
     ap.add_argument("--workload", default="batch",
                     choices=["batch", "uneven", "precision", "serve", "copy", "config0"],
                     help="batch: 4096 x N=1024, S=256 (default, the headline); uneven: 20000 systems of N = 2^U{8..14} "
@@ -424,22 +427,23 @@ def precision_workload(args):
         return e0.elapsed_time(e1) / reps
 
     variants = []
-    for coef16 in (False, True):
-        for steps in (0, 1, 2):
-            ms = timed(lambda: solve(coef16, steps), 10)
-            solve(coef16, steps)
-            torch.cuda.synchronize()
-            x = t["x"].cpu().numpy()
-            # algorithmic bytes: solve 20 B/eq (14 with fp16 a, b, c), each
-            # refinement step 24 B/eq (a, b, c, d, x read; x written)
-            alg = nsys * n * ((14 if coef16 else 20) + 24 * steps)
-            variants.append({"name": "%s%s" % ("fp16" if coef16 else "fp32", "+%d refine" % steps if steps else ""),
-                             "ms": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
-                             "GBps": round(alg / (ms / 1e3) / 1e9, 1), "x": x})
-    ms = timed(per_system, 2)
-    variants.append({"name": "fp32, one system per launch", "ms": round(ms, 4),
-                     "systems_per_s": round(nsys / (ms / 1e3), 1),
-                     "GBps": round(20 * nsys * n / (ms / 1e3) / 1e9, 1), "x": None})
+    combos = [(False, 0)] if args.fp32_only else [(c16, st) for c16 in (False, True) for st in (0, 1, 2)]
+    for coef16, steps in combos:
+        ms = timed(lambda: solve(coef16, steps), 10)
+        solve(coef16, steps)
+        torch.cuda.synchronize()
+        x = t["x"].cpu().numpy()
+        # algorithmic bytes: solve 20 B/eq (14 with fp16 a, b, c), each
+        # refinement step 24 B/eq (a, b, c, d, x read; x written)
+        alg = nsys * n * ((14 if coef16 else 20) + 24 * steps)
+        variants.append({"name": "%s%s" % ("fp16" if coef16 else "fp32", "+%d refine" % steps if steps else ""),
+                         "ms": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
+                         "GBps": round(alg / (ms / 1e3) / 1e9, 1), "x": x})
+    if not args.fp32_only:
+        ms = timed(per_system, 2)
+        variants.append({"name": "fp32, one system per launch", "ms": round(ms, 4),
+                         "systems_per_s": round(nsys / (ms / 1e3), 1),
+                         "GBps": round(20 * nsys * n / (ms / 1e3) / 1e9, 1), "x": None})
     host = {k: t[k].cpu().numpy() for k in ("a", "b", "c", "d")}
     head = variants[0]  # the fp32 solve is the line's value; the rest are variants of it
     out = {
@@ -455,6 +459,13 @@ def precision_workload(args):
                      "frac": round(head["GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
         "variants": variants,
     }
+    # bytes past L2 per fp32 solve (the three kernels) from FETCH_SIZE /
+    # WRITE_SIZE runs, scaled by the copy calibration: the stage-3 re-read
+    # makes it ~1.8x the algorithmic bytes
+    tf = os.path.join(ROOT, "profiles", "traffic_ptpr_split256x65536.json")
+    if ch == 0 and os.path.exists(tf):
+        with open(tf) as f:
+            out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_step")
     if not args.no_cpu:
         import oracle as O
         xf = O.thomas_f64_batch(host["a"], host["b"], host["c"], host["d"], n).reshape(nsys, n)
