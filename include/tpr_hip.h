@@ -281,9 +281,11 @@ typedef struct tprh_request_info {
 
 tprh_status tprh_server_request_info(tprh_server *srv, long long request, tprh_request_info *info);
 /* Blocks until the request is done (TPRH_SUCCESS), failed (its error), or
- * timeout_s passes (TPRH_ERROR_TIMEOUT; the request keeps going). */
+ * timeout_s passes (TPRH_ERROR_TIMEOUT; the request keeps going).  timeout_s
+ * must be >= 0 (INFINITY is allowed; waits beyond a year end after a year). */
 tprh_status tprh_server_wait(tprh_server *srv, long long request, double timeout_s);
-/* Ids of requests completed since the previous call (at most max). */
+/* Ids of requests completed since the previous call (at most max), oldest
+ * first; requests already released are left out. */
 tprh_status tprh_server_poll(tprh_server *srv, long long *ids, int max, int *count);
 /* As tprh_server_poll, but sleeps until at least one request has completed
  * or timeout_s passes (then *count is 0).  Lets a client wait for whichever

@@ -222,6 +222,42 @@ def test_serve_wait_any():
         srv.matrix_release(m)
 
 
+def test_serve_poll_skips_released_and_timeouts():
+    """Clients that wait and release without polling: poll reports only the
+    requests still held, also after more than 4096 such completions (the
+    completion list is pruned of released ids); wait refuses a NaN or
+    negative timeout and takes an infinite one."""
+    say("serve poll after wait/release")
+    with Server(64 << 20, max_inflight=4, max_batch_systems=256) as srv:
+        m = srv.matrix_generate(512, seed=SEED, sys=4300)
+        done = 0
+        while done < 4400:
+            rids = [srv.submit(m, 1, rhs_seed=SEED, j0=done + q) for q in range(200)]
+            for r in rids:
+                srv.wait(r)
+                srv.release(r)
+            done += len(rids)
+        keep = [srv.submit(m, 2, rhs_seed=SEED, j0=q) for q in range(3)]
+        srv.wait(keep[0], timeout=float("inf"))
+        for r in keep:
+            srv.wait(r)
+        srv.release(keep[1])
+        got = []
+        while True:
+            ids = srv.poll(timeout=0.05)
+            if not ids:
+                break
+            got += ids
+        assert sorted(got) == sorted([keep[0], keep[2]])
+        L = _lib.lib()
+        assert L.tprh_server_wait(srv._h, keep[0], float("nan")) == _lib.ERROR_INVALID_VALUE
+        assert L.tprh_server_wait(srv._h, keep[0], -1.0) == _lib.ERROR_INVALID_VALUE
+        srv.release(keep[0])
+        srv.release(keep[2])
+        srv.matrix_release(m)
+        assert srv.stats()["completed"] == 4400 + 3
+
+
 def test_serve_concurrent_clients():
     """Four client threads create matrices, submit, wait and release at the
     same time (the library calls release the interpreter lock): every answer

@@ -595,6 +595,14 @@ void worker_main(tprh_server* sv) {
     }
 }
 
+// A wait's deadline on the steady clock; timeouts beyond a year are taken as
+// a year, so that the conversion to the clock's integer ticks cannot overflow.
+std::chrono::steady_clock::time_point wait_deadline(double timeout_s) {
+    const double t = std::min(timeout_s, 365.0 * 86400.0);
+    return std::chrono::steady_clock::now() +
+           std::chrono::duration_cast<std::chrono::steady_clock::duration>(std::chrono::duration<double>(t));
+}
+
 tprh_status validate_n(tprh_server* sv, int n, int* s, int* E, int* L, bool* split) {
     tprh_status st = tprh::class_shape(n, sv->have_schedule ? sv->schedule : nullptr, s, E, L, true, split,
                                        (int)sv->cfg.method, (int)sv->cfg.arith);
@@ -796,6 +804,7 @@ tprh_status tprh_server_matrix_upload(tprh_server* sv, int n, const float* a, co
         (e = hipMemcpyAsync(dst + n, b, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
         (e = hipMemcpyAsync(dst + 2 * (size_t)n, c, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
         (e = hipEventRecord(ev, sv->upload)) != hipSuccess || (e = hipEventSynchronize(ev)) != hipSuccess) {
+        (void)hipStreamSynchronize(sv->upload);  // copies already queued finish before the block is reused
         std::lock_guard<std::mutex> g(sv->mu);
         free_matrix_locked(sv, *matrix);
         return tprh::set_hip_error(e, "matrix upload");
@@ -886,8 +895,9 @@ tprh_status tprh_server_request_info(tprh_server* sv, long long request, tprh_re
 
 tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_s) {
     if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
+    if (!(timeout_s >= 0.0)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "timeout must be >= 0");
     std::unique_lock<std::mutex> lk(sv->mu);
-    const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    const auto deadline = wait_deadline(timeout_s);
     for (;;) {
         auto it = sv->reqs.find(request);
         if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
@@ -901,10 +911,16 @@ tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_
     }
 }
 
+// Hands out up to max completed ids, oldest first; ids of requests already
+// released are dropped on the way.
 static void take_completed_locked(tprh_server* sv, long long* ids, int max, int* count) {
-    const int k = std::min<int>(max, (int)sv->completed.size());
-    for (int i = 0; i < k; i++) ids[i] = sv->completed[i];
-    sv->completed.erase(sv->completed.begin(), sv->completed.begin() + k);
+    int k = 0;
+    size_t i = 0;
+    for (; i < sv->completed.size() && k < max; i++) {
+        const long long id = sv->completed[i];
+        if (sv->reqs.count(id)) ids[k++] = id;
+    }
+    sv->completed.erase(sv->completed.begin(), sv->completed.begin() + (long)i);
     *count = k;
 }
 
@@ -919,9 +935,13 @@ tprh_status tprh_server_wait_any(tprh_server* sv, long long* ids, int max, int* 
     if (!sv || !count || max < 1 || !ids) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument or max < 1");
     if (!(timeout_s >= 0.0)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "timeout must be >= 0");
     std::unique_lock<std::mutex> lk(sv->mu);
-    const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
-    sv->cv_done.wait_until(lk, deadline, [sv] { return !sv->completed.empty(); });
-    take_completed_locked(sv, ids, max, count);
+    const auto deadline = wait_deadline(timeout_s);
+    *count = 0;
+    // the list may hold only ids released meanwhile: then wait on
+    while (sv->cv_done.wait_until(lk, deadline, [sv] { return !sv->completed.empty(); })) {
+        take_completed_locked(sv, ids, max, count);
+        if (*count > 0) break;
+    }
     return TPRH_SUCCESS;
 }
 
@@ -956,6 +976,13 @@ tprh_status tprh_server_request_release(tprh_server* sv, long long request) {
         sv->cv_work.notify_one();  // queued requests may fit now
     }
     sv->reqs.erase(it);
+    // clients that wait and release without polling leave their ids in the
+    // completion list: drop the released ones once the list outgrows the live
+    // requests (amortised O(1) per release)
+    if (sv->completed.size() > 2 * sv->reqs.size() + 4096)
+        sv->completed.erase(std::remove_if(sv->completed.begin(), sv->completed.end(),
+                                           [sv](long long id) { return sv->reqs.count(id) == 0; }),
+                            sv->completed.end());
     return TPRH_SUCCESS;
 }
 
