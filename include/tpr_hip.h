@@ -187,7 +187,9 @@ tprh_status tprh_solve_ex(const tprh_solve_desc *desc, void *stream);
  * class's entry only, never on the rest of the batch.  Sizes: powers of two
  * from 16 with n / s * L <= 1024.  Creating the plan allocates its device
  * tables; tprh_uneven_solve allocates nothing and is graph-capturable (two
- * kernels: a counter reset and the persistent solver). */
+ * kernels: a counter reset and the persistent solver).  A plan holds one work
+ * counter, so its solves must not overlap on the device: issue them on one
+ * stream (or order them with events); solves that overlap need a plan each. */
 typedef struct tprh_uneven_plan tprh_uneven_plan;
 tprh_status tprh_uneven_plan_create(tprh_uneven_plan **out, tprh_method method, tprh_arith arith, const int *n,
                                     int nsys, const int *schedule);
