@@ -36,9 +36,11 @@ def test_cpu_solver_names_are_refused(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,s", [("cuptpr", 64), ("CUPTPR", 1024), ("cutpr", 256), ("hipptpr", 128),
-                                    ("hiptpr", 512), ("hippcr", 0), ("hipcr", 0)])
+                                    ("hiptpr", 512), ("hippcr", 0), ("hipcr", 0), ("cusparse", 0),
+                                    ("rocsparse", 0)])
 def test_known_answer_through_the_driver(golden, tmp_path, name, s):
-    """TPR_PM_INPUT=fixed: the system of test/ans1024.txt; x written with %f."""
+    """TPR_PM_INPUT=fixed: the system of test/ans1024.txt; x written with %f.
+    cusparse / rocsparse: the vendor slot (src/pm.cpp:240-246) on rocSPARSE."""
     out = tmp_path / "x.txt"
     p = run([1024, s, 3, name, out], env={"TPR_PM_INPUT": "fixed"})
     assert p.returncode == 0, p.stderr

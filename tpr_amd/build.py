@@ -75,8 +75,9 @@ def build_cli():
     if not os.path.exists(CLI_SRC):
         return None
     tmp = CLI_BIN + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), CLI_SRC, "-o", tmp,
-          "-L" + HERE, "-ltpr_hip", "-Wl,-rpath,$ORIGIN"])
+    _run(["g++", "-O2", "-std=c++17"] + HOST_HIP + ["-I" + os.path.join(ROOT, "include"), CLI_SRC, "-o", tmp,
+          "-L" + HERE, "-ltpr_hip", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-ldl",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, CLI_BIN)
     return CLI_BIN
 

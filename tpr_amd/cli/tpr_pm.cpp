@@ -8,8 +8,10 @@
 //   cutpr,  hiptpr    TPR on the GPU    (TPR_CU::tpr_cu)
 //   hippcr            PCR on the GPU    (PTPR_CU::pcr_cu), n <= 4096
 //   hipcr             CR on the GPU     (TPR_CU::cr_cu),   n <= 4096
-// The reference spellings cuptpr / cutpr are names for the one HIP
-// implementation; there is no CUDA code behind them.  The reference's CPU
+//   cusparse, rocsparse  the vendor slot (REFERENCE_CUSPARSE, src/pm.cpp:240-246):
+//                     rocSPARSE's gtsv without pivoting, from libtpr_rocsparse.so
+// The reference spellings cuptpr / cutpr / cusparse are names for the HIP and
+// rocSPARSE paths; there is no CUDA code behind them.  The reference's CPU
 // solvers (tpr, pcr, ptpr, lapack) are not part of this GPU engine.
 //
 // Protocol of src/pm.cpp:226-239: per iteration the input is regenerated
@@ -24,12 +26,16 @@
 // solver) or fast; TPR_PM_INPUT=random (default) or fixed, the reference's
 // ExampleFixedInput (rhs = i + 1, src/system.cpp:118-130) whose answer for
 // n = 1024 is test/ans1024.txt.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -37,7 +43,7 @@
 
 namespace {
 
-enum class Solver { PTPR, TPR, PCR, CR };
+enum class Solver { PTPR, TPR, PCR, CR, VENDOR };
 
 bool parse_solver(std::string name, Solver* out) {
     std::transform(name.begin(), name.end(), name.begin(), [](unsigned char ch) { return std::tolower(ch); });
@@ -45,6 +51,7 @@ bool parse_solver(std::string name, Solver* out) {
     else if (name == "cutpr" || name == "hiptpr") *out = Solver::TPR;
     else if (name == "hippcr") *out = Solver::PCR;
     else if (name == "hipcr") *out = Solver::CR;
+    else if (name == "cusparse" || name == "rocsparse") *out = Solver::VENDOR;
     else if (name == "tpr" || name == "pcr" || name == "ptpr" || name == "lapack") {
         std::cerr << "Solver `" << name << "` is a CPU solver of the reference and is not part of this GPU engine; "
                   << "use cuptpr/hipptpr, cutpr/hiptpr, hippcr or hipcr.\n";
@@ -74,6 +81,65 @@ struct Input {  // ExampleRandomRHSInput (src/system.cpp:146-158) / ExampleFixed
     std::uniform_real_distribution<double> dist;
 };
 
+// The vendor slot as the reference runs it (src/reference_cusparse.cu:164-198):
+// per call the inputs are copied to the device, the gtsv call alone is timed
+// with device events, and x is copied back.  The library is looked up beside
+// libtpr_hip.so at run time, as tpr_nstab does.
+class VendorSlot {
+public:
+    explicit VendorSlot(int n) : n_(n) {
+        Dl_info info;
+        std::string dir = ".";
+        if (dladdr(reinterpret_cast<void*>(&tprh_version), &info) && info.dli_fname) {
+            std::string f = info.dli_fname;
+            const size_t k = f.find_last_of('/');
+            if (k != std::string::npos) dir = f.substr(0, k);
+        }
+        void* h = dlopen((dir + "/libtpr_rocsparse.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h) throw std::runtime_error(std::string("the rocSPARSE slot needs libtpr_rocsparse.so: ") + dlerror());
+        fn_ = reinterpret_cast<Fn>(dlsym(h, "tprh_vendor_gtsv_batch"));
+        err_ = reinterpret_cast<const char* (*)()>(dlsym(h, "tprh_vendor_last_error"));
+        if (!fn_ || !err_) throw std::runtime_error("libtpr_rocsparse.so lacks tprh_vendor_gtsv_batch");
+        if (n <= 0) throw std::runtime_error("n must be positive");
+        hip(hipMalloc(reinterpret_cast<void**>(&dev_), 5 * sizeof(float) * (size_t)n), "hipMalloc");
+        hip(hipEventCreate(&ev_[0]), "hipEventCreate");
+        hip(hipEventCreate(&ev_[1]), "hipEventCreate");
+    }
+    ~VendorSlot() {
+        for (hipEvent_t e : ev_)
+            if (e) (void)hipEventDestroy(e);
+        if (dev_) (void)hipFree(dev_);
+    }
+    // a, diag, c, rhs: host arrays of n; x: host; returns the gtsv time in ms
+    float solve(const float* a, const float* diag, const float* c, const float* rhs, float* x) {
+        const size_t nb = sizeof(float) * (size_t)n_;
+        const float* src[4] = {a, diag, c, rhs};
+        for (int k = 0; k < 4; k++) hip(hipMemcpy(dev_ + (size_t)k * n_, src[k], nb, hipMemcpyHostToDevice), "hipMemcpy");
+        hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip(hipEventRecord(ev_[0], nullptr), "hipEventRecord");
+        const int st = fn_(dev_, dev_ + n_, dev_ + 2 * (size_t)n_, dev_ + 3 * (size_t)n_, dev_ + 4 * (size_t)n_, n_, 1,
+                           nullptr);
+        if (st != 0) throw std::runtime_error(std::string("rocSPARSE gtsv: ") + err_());
+        hip(hipEventRecord(ev_[1], nullptr), "hipEventRecord");
+        hip(hipEventSynchronize(ev_[1]), "hipEventSynchronize");
+        float ms = -1.0f;
+        hip(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+        hip(hipMemcpy(x, dev_ + 4 * (size_t)n_, nb, hipMemcpyDeviceToHost), "hipMemcpy");
+        return ms;
+    }
+
+private:
+    using Fn = int (*)(const float*, const float*, const float*, const float*, float*, int, int, void*);
+    static void hip(hipError_t e, const char* what) {
+        if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+    }
+    int n_;
+    Fn fn_ = nullptr;
+    const char* (*err_)() = nullptr;
+    float* dev_ = nullptr;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
@@ -98,24 +164,34 @@ int main(int argc, char* argv[]) {
     std::vector<float> times;
     times.reserve(iter > 0 ? iter : 0);
     try {
-        tpr::hip::SolverHandle* h = nullptr;
-        switch (solver) {
-            case Solver::PTPR: h = new tpr::hip::PTPR(n, s); break;
-            case Solver::TPR: h = new tpr::hip::TPR(n, s); break;
-            case Solver::PCR: h = new tpr::hip::PCR(n); break;
-            case Solver::CR: h = new tpr::hip::CR(n); break;
+        if (solver == Solver::VENDOR) {  // src/pm.cpp:240-246: the diagonal is 1 (the 3-array input)
+            VendorSlot vs(n);
+            std::vector<float> ones(n > 0 ? n : 0, 1.0f);
+            for (int i = 0; i < iter; i++) {
+                input.assign();
+                times.push_back(vs.solve(input.a.data(), ones.data(), input.c.data(), input.rhs.data(),
+                                         input.diag.data()));
+            }
+        } else {
+            tpr::hip::SolverHandle* h = nullptr;
+            switch (solver) {
+                case Solver::PTPR: h = new tpr::hip::PTPR(n, s); break;
+                case Solver::TPR: h = new tpr::hip::TPR(n, s); break;
+                case Solver::PCR: h = new tpr::hip::PCR(n); break;
+                default: h = new tpr::hip::CR(n); break;
+            }
+            h->set_arith(arith);
+            tpr::hip::check(tprh_set_kernel_timing(1), "tprh_set_kernel_timing");
+            for (int i = 0; i < iter; i++) {
+                input.assign();
+                h->set_tridiagonal_system(input.a.data(), input.c.data(), input.rhs.data());
+                h->solve();
+                h->get_ans(input.diag.data());  // the reference returns x in sys.diag
+                times.push_back(tprh_last_kernel_ms());
+            }
+            delete h;
         }
-        h->set_arith(arith);
-        tpr::hip::check(tprh_set_kernel_timing(1), "tprh_set_kernel_timing");
-        for (int i = 0; i < iter; i++) {
-            input.assign();
-            h->set_tridiagonal_system(input.a.data(), input.c.data(), input.rhs.data());
-            h->solve();
-            h->get_ans(input.diag.data());  // the reference returns x in sys.diag
-            times.push_back(tprh_last_kernel_ms());
-        }
-        delete h;
-    } catch (const tpr::hip::Error& e) {
+    } catch (const std::exception& e) {  // tpr::hip::Error and the vendor slot's errors
         std::cerr << "[Error] " << e.what() << "\n";
         return EXIT_FAILURE;
     }
