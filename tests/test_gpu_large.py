@@ -63,6 +63,28 @@ def test_split_exact_bit_identical(method, n, s, nsys, chunk):
 
 
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s", [(1 << 22, 2048), (1 << 19, 256)])
+def test_split_maximum_sizes(method, n, s):
+    """The largest systems the engine takes: m = n / s = 2048 slices (the
+    multi-workgroup path's limit) at the largest slice size, 4 Mi equations,
+    and at s = 256; exact answers bit-identical, fast within tolerance, and
+    one slice more is refused."""
+    say("split maximum", method, n, s)
+    a, b, c, d, xt = O.gen_batch(33, 0, 1, n, O.GEN_WEAK)
+    ref = O.solve_batch(method, a, b, c, d, n, s)
+    x = torch.empty(n, device="cuda")
+    da, db, dc, dd = dev(a), dev(b), dev(c), dev(d)
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, arith="exact")
+    torch.cuda.synchronize()
+    assert bits_equal(x.cpu().numpy(), ref)
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, arith="fast")
+    torch.cuda.synchronize()
+    assert per_system_rel(x.cpu().numpy(), ref, n).max() <= FAST_TOL
+    with pytest.raises(tpr_amd.TprError):
+        B.solve_ex(da, db, dc, dd, x, 2 * n, s, method=method, arith="exact")
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
 @pytest.mark.parametrize("n,s,nsys,chunk", SPLIT_CASES[:4])
 def test_split_fast_within_tolerance(method, n, s, nsys, chunk):
     say("split fast", method, n, s, nsys, chunk)
