@@ -63,7 +63,7 @@ bool parse_solver(std::string name, Solver* out) {
 }
 
 struct Input {  // ExampleRandomRHSInput (src/system.cpp:146-158) / ExampleFixedInput
-    Input(int n, bool fixed) : n(n), fixed(fixed), a(n), diag(n), c(n), rhs(n), rng(0), dist(-1.0, 1.0) {}
+    Input(int n_, bool fixed_) : n(n_), fixed(fixed_), a(n_), diag(n_), c(n_), rhs(n_), rng(0), dist(-1.0, 1.0) {}
     void assign() {
         for (int i = 0; i < n; i++) {
             a[i] = static_cast<float>(-1.0 / 6.0);
