@@ -15,7 +15,12 @@
 // (under ThreadSanitizer) that every host access is ordered by the server's
 // locks and the events it waits on.
 //
-//   server_stress <client threads> <requests per client> <seed>
+// With a fourth argument k > 0, every k-th stub launch (serve, split or
+// generation) returns an error instead of queueing its work: the requests of
+// that batch must fail with TPRH_ERROR_HIP, their memory must come back, and
+// every other request must still be answered correctly.
+//
+//   server_stress <client threads> <requests per client> <seed> [fail every k-th launch]
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +45,10 @@
 #include "../../tpr_amd/csrc/launch_split.hpp"
 
 static std::atomic<int> g_failures{0};
+static int g_fail_every = 0;                  // 0: no injected launch failures
+static std::atomic<long long> g_launches{0};  // stub launches so far
+
+static bool inject_failure() { return g_fail_every > 0 && (g_launches.fetch_add(1) + 1) % g_fail_every == 0; }
 static std::mutex g_print;
 
 #define FAIL(...)                                      \
@@ -239,6 +248,7 @@ static int shape_n(const UnevenTask& t) {
 }
 
 hipError_t launch_serve(int, int, const ServeLaunch& p, hipStream_t stream) {
+    if (inject_failure()) return hipErrorUnknown;
     const ServeLaunch q = p;
     stream->push([q] {
         random_pause(300);
@@ -264,6 +274,7 @@ hipError_t launch_serve(int, int, const ServeLaunch& p, hipStream_t stream) {
 }
 
 hipError_t launch_serve_split(int, int, const ServeSplitLaunch& p, int E, int L, hipStream_t stream) {
+    if (inject_failure()) return hipErrorUnknown;
     const ServeSplitLaunch q = p;
     if (!serve_split_shape(E, L) || p.n % (E * L)) FAIL("split launch: shape %dx%d for n=%d", E, L, p.n);
     const int m = p.n / (E * L);
@@ -285,6 +296,7 @@ hipError_t launch_serve_split(int, int, const ServeSplitLaunch& p, int E, int L,
 }
 
 hipError_t launch_generate_jobs(const GenJob* jobs, int njobs, long long total, float* base, hipStream_t stream) {
+    if (inject_failure()) return hipErrorUnknown;
     stream->push([jobs, njobs, total, base] {
         random_pause(100);
         long long at = 0;
@@ -329,7 +341,8 @@ static void check_ok(tprh_status st, const char* what) {
     if (st != TPRH_SUCCESS) FAIL("%s: status %d (%s)", what, (int)st, t_err.c_str());
 }
 
-static void client(tprh_server* sv, int tid, int iters, uint64_t seed, std::atomic<long long>* solved) {
+static void client(tprh_server* sv, int tid, int iters, uint64_t seed, std::atomic<long long>* solved,
+                   std::atomic<long long>* failed) {
     std::mt19937_64 rng(seed * 1000 + tid);
     const int sizes[] = {16, 64, 256, 1024, 4096, 16384, 32768, 65536};
     for (int it = 0; it < iters; it++) {
@@ -389,9 +402,16 @@ static void client(tprh_server* sv, int tid, int iters, uint64_t seed, std::atom
         }
         for (Req& r : reqs) {
             if (r.id <= 0) continue;
-            check_ok(tprh_server_wait(sv, r.id, 120.0), "wait");
+            const tprh_status ws = tprh_server_wait(sv, r.id, 120.0);
             tprh_request_info inf;
             check_ok(tprh_server_request_info(sv, r.id, &inf), "info");
+            if (ws == TPRH_ERROR_HIP && g_fail_every > 0) {  // its batch's launch was made to fail
+                if (inf.state != 3 || inf.x) FAIL("failed request %lld: state %d", r.id, inf.state);
+                check_ok(tprh_server_request_release(sv, r.id), "release (failed)");
+                failed->fetch_add(1);
+                continue;
+            }
+            check_ok(ws, "wait");
             if (inf.state != 2) FAIL("request %lld state %d after wait", r.id, inf.state);
             std::vector<float> got;
             if (r.host_x) {
@@ -427,6 +447,7 @@ int main(int argc, char** argv) {
     const int threads = argc > 1 ? std::atoi(argv[1]) : 4;
     const int iters = argc > 2 ? std::atoi(argv[2]) : 40;
     const uint64_t seed = argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 1;
+    g_fail_every = argc > 4 ? std::atoi(argv[4]) : 0;
 
     tprh_server_config cfg{};
     cfg.method = TPRH_PTPR;
@@ -440,7 +461,7 @@ int main(int argc, char** argv) {
     check_ok(tprh_server_create(&sv, &cfg), "create");
     if (!sv) return 2;
 
-    std::atomic<long long> solved{0};
+    std::atomic<long long> solved{0}, failed{0};
     std::atomic<bool> done{false};
     // a thread that drains the completion list the whole time (ids it sees
     // may already be released: it only counts them) and one that reads stats
@@ -460,7 +481,7 @@ int main(int argc, char** argv) {
         }
     });
     std::vector<std::thread> cl;
-    for (int t = 0; t < threads; t++) cl.emplace_back(client, sv, t, iters, seed, &solved);
+    for (int t = 0; t < threads; t++) cl.emplace_back(client, sv, t, iters, seed, &solved, &failed);
     for (auto& t : cl) t.join();
     done = true;
     poller.join();
@@ -468,12 +489,17 @@ int main(int argc, char** argv) {
 
     tprh_server_stats s;
     check_ok(tprh_server_stats_get(sv, &s), "stats");
-    if (s.failed != 0) FAIL("%lld requests failed", s.failed);
+    if (s.failed != failed.load()) FAIL("%lld requests failed, clients saw %lld", s.failed, failed.load());
+    if (g_fail_every == 0 && s.failed != 0) FAIL("%lld requests failed", s.failed);
+    if (g_fail_every > 0 && s.failed == 0) FAIL("no injected failure reached a request");
+    if (s.completed + s.failed != s.submitted)
+        FAIL("%lld submitted, %lld completed, %lld failed", s.submitted, s.completed, s.failed);
     if (s.bytes_in_use != 0) FAIL("%zu arena bytes still in use after every release", s.bytes_in_use);
     if (s.systems != solved.load()) FAIL("server solved %lld systems, clients checked %lld", s.systems, solved.load());
     check_ok(tprh_server_destroy(sv), "destroy");
-    std::printf("server_stress: %d clients, %lld requests, %lld systems, %lld batches, max in flight %d, %s\n",
-                threads, s.completed, s.systems, s.batches, s.max_inflight_seen,
+    std::printf("server_stress: %d clients, %lld requests (%lld failed), %lld systems, %lld batches, max in flight %d, "
+                "%s\n",
+                threads, s.completed, s.failed, s.systems, s.batches, s.max_inflight_seen,
                 g_failures.load() ? "FAILED" : "ok");
     return g_failures.load() ? 1 : 0;
 }

@@ -10,13 +10,18 @@ threads, a polling thread and a stats thread against a tight memory budget.
   another), with and without the batching window;
 * every answer is the exact value the stub rule gives, every request
   completes, the arena is empty once everything is released, and the
-  server's count of solved systems equals the clients'.
+  server's count of solved systems equals the clients';
+* failure handling: with every k-th launch made to fail on the host, exactly
+  the requests of those batches fail (TPRH_ERROR_HIP, state "failed", no
+  answers), their memory comes back and the server goes on answering the
+  rest correctly -- a failure path no GPU test may force.
 
 LLVM's ThreadSanitizer (ROCm's clang) is used: GCC 11's runtime does not
 intercept pthread_cond_clockwait, which libstdc++ uses for timed waits on the
 steady clock, and then reports every access under the server's mutex.
 """
 import os
+import re
 import shutil
 import subprocess
 
@@ -37,11 +42,11 @@ def build(out, sanitize):
     assert p.returncode == 0, p.stderr[-4000:]
 
 
-def run(exe, clients, iters, seed, sanitize):
+def run(exe, clients, iters, seed, sanitize, fail_every=0):
     env = dict(os.environ)
     env["TSAN_OPTIONS"] = "halt_on_error=1 exitcode=66 second_deadlock_stack=1"
-    p = subprocess.run([str(exe), str(clients), str(iters), str(seed)], capture_output=True, text=True, env=env,
-                       timeout=300)
+    p = subprocess.run([str(exe), str(clients), str(iters), str(seed), str(fail_every)], capture_output=True,
+                       text=True, env=env, timeout=300)
     if sanitize and "unexpected memory mapping" in p.stderr:
         pytest.skip("ThreadSanitizer cannot map its shadow memory on this kernel")
     assert p.returncode == 0, (p.stdout + p.stderr)[-6000:]
@@ -63,6 +68,13 @@ def tsan_exe(tmp_path_factory):
 def test_server_threads_under_tsan(tsan_exe, seed):
     out = run(tsan_exe, 4, 40, seed, sanitize=True)
     assert "max in flight" in out
+
+
+@pytest.mark.parametrize("seed,fail_every", [(2, 7), (1, 3)])
+def test_server_launch_failures_under_tsan(tsan_exe, seed, fail_every):
+    out = run(tsan_exe, 4, 40, seed, sanitize=True, fail_every=fail_every)
+    failed = int(re.search(r"\((\d+) failed\)", out).group(1))
+    assert failed > 0, out
 
 
 def test_server_stress_plain_build(tmp_path):
