@@ -120,12 +120,12 @@ tprh_status tprh_solver_destroy(tprh_solver *h);
  * [k*n, (k+1)*n) of a, b, c, d and x, all device pointers.  b == NULL means a
  * unit diagonal.  Asynchronous on `stream` (a hipStream_t; NULL = the null
  * stream); no host synchronisation, no allocation, graph-capturable.
- * method: TPRH_PTPR or TPRH_TPR.  The slice shape is the default for s,
- * widened where that measured faster without changing a result: s = 512
- * takes 8 equations per lane x 64 lanes while n / s <= 8 (PTPR) or <= 4
- * (TPR); exact PTPR at s = 256 takes 4 x 64 while n / s <= 16, exact TPR
- * at s = 256 takes 8 x 32 in batches of at most 256 systems while
- * n / s <= 8. */
+ * method: TPRH_PTPR or TPRH_TPR.  The slice shape is the default for s
+ * (16 lanes per slice up to s = 256, then 32 or 64) except where a tuned
+ * table measured on MI355X names a faster one that gives the same answers:
+ * s = 256 and 512, n / s <= 16, by method, arithmetic and batch size (the
+ * tuned batch size nearest to nsys).  Answers never depend on the batch
+ * size. */
 tprh_status tprh_solve_batched(tprh_method method, tprh_arith arith, const float *a, const float *b,
                                const float *c, const float *d, float *x, int n, int s, int nsys,
                                void *stream);

@@ -123,6 +123,47 @@ def test_alternative_shapes_bit_identical(s, E, L, method):
     assert bits_equal(x, O.solve_batch(method, a, b, c, d, n, s))
 
 
+@pytest.mark.parametrize("m", [1, 4, 16])
+@pytest.mark.parametrize("s", [256, 512])
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_every_tuner_candidate_both_modes(method, s, m):
+    """Every slice shape the batched tuner times (E * L = s, one workgroup
+    per system) up to 16 slices per system: exact answers bit-identical to
+    the oracle, fast answers within 1e-5."""
+    from tpr_amd.autotune import batched_candidates
+    n, nsys = s * m, 3
+    a, b, c, d, _ = O.gen_batch(5, 0, nsys, n)
+    ref = O.solve_batch(method, a, b, c, d, n, s)
+    for shape in batched_candidates(n, s):
+        say("candidate", method, n, s, shape)
+        assert bits_equal(gpu_solve(a, b, c, d, n, s, method, "exact", shape=shape), ref)
+        x = gpu_solve(a, b, c, d, n, s, method, "fast", shape=shape)
+        assert per_system_rel(x, ref, n).max() <= FAST_TOL
+
+
+def _tuned_rows_off_default():
+    from tpr_amd import autotune as A
+    out = []
+    for key, row in sorted(A.load(A.BATCHED_PATH)["cases"].items()):
+        method, arith, n, s, nsys = A.parse_batched_key(key)
+        if tuple(row["choice"]) != A.default_batched_shape(s):
+            out.append((method, arith, n, s, tuple(row["choice"]), A.default_batched_shape(s)))
+    return sorted(set(out))
+
+
+@pytest.mark.parametrize("method,arith,n,s,shape,default", _tuned_rows_off_default())
+def test_tuned_default_shapes_keep_the_answers(method, arith, n, s, shape, default):
+    """Every shape the tuned table makes a default (where it is not the
+    library's plain default shape) gives the plain default's bits, in fast
+    arithmetic too, on four input kinds: the table changes speed only, never
+    a result, whatever the batch size."""
+    for kind in (O.GEN_DOMINANT, O.GEN_WEAK, O.GEN_FIXED, O.GEN_RANDOM_RHS):
+        say("tuned shape", method, arith, n, s, shape, kind)
+        a, b, c, d, _ = O.gen_batch(21 + kind, 0, 3, n, kind)
+        x = gpu_solve(a, b, c, d, n, s, method, arith, shape=shape)
+        assert bits_equal(x, gpu_solve(a, b, c, d, n, s, method, arith, shape=default))
+
+
 @pytest.mark.parametrize("n", [1024, 2048, 4096, 8192])
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
 def test_s512_default_shape_matches_16x32(method, n):

@@ -1,37 +1,96 @@
-"""The default slice shape (tprh_slice_shape, no device call): the
-measured rule of wider shapes where they are faster without changing a
-result, and the default shape everywhere else."""
+"""The default slice shape of the batched entry points (tprh_slice_shape, no
+device call): the rows of the committed tuner table
+(tpr_amd/schedules/batched_gfx950.json, compiled into the library from
+tpr_amd/csrc/batched_shapes.inc), the nearest tuned batch size for other
+batch sizes, and the default shape wherever the table has no row."""
+import json
+import os
+
 import pytest
 
 from tpr_amd import _lib
+from tpr_amd import autotune as A
 from tpr_amd.batched import slice_shape
 
-CASES = [
-    # (n, s, nsys, method, arith) -> (E, L)
-    ((1024, 256, 4096, "ptpr", "fast"), (16, 16)),     # config 1: the default
-    ((1024, 256, 1, "ptpr", "fast"), (16, 16)),        # fast answers would change with 4 x 64
-    ((1024, 256, 4096, "tpr", "fast"), (16, 16)),
-    ((1024, 256, 4096, "ptpr", "exact"), (4, 64)),     # exact PTPR, m <= 16
-    ((4096, 256, 7, "ptpr", "exact"), (4, 64)),
-    ((8192, 256, 7, "ptpr", "exact"), (16, 16)),       # m = 32: 4 x 64 would need 2048 lanes
-    ((1024, 256, 256, "tpr", "exact"), (8, 32)),       # exact TPR, small batches
-    ((1024, 256, 257, "tpr", "exact"), (16, 16)),
-    ((4096, 256, 1, "tpr", "exact"), (16, 16)),        # m = 16
-    ((2048, 512, 1, "ptpr", "fast"), (8, 64)),         # s = 512: PTPR up to m = 8
-    ((4096, 512, 4096, "ptpr", "exact"), (8, 64)),
-    ((8192, 512, 1, "ptpr", "fast"), (16, 32)),
-    ((2048, 512, 1, "tpr", "fast"), (8, 64)),          # TPR up to m = 4
-    ((4096, 512, 1, "tpr", "fast"), (16, 32)),
-    ((512, 128, 1, "ptpr", "exact"), (8, 16)),         # other s: default_shape
-    ((4096, 1024, 1, "ptpr", "fast"), (16, 64)),
-    ((16, 16, 1, "tpr", "fast"), (1, 16)),
-]
+TABLE = A.load(A.BATCHED_PATH)
+ROWS = sorted(TABLE["cases"].items())
 
 
-@pytest.mark.parametrize("args,shape", CASES)
-def test_rule(args, shape):
-    n, s, nsys, method, arith = args
-    assert slice_shape(n, s, nsys, method, arith) == shape
+def test_header_is_the_table():
+    """The rows compiled into the library are the committed table's."""
+    with open(A.HEADER_PATH) as f:
+        assert f.read() == A.header_text(TABLE)
+
+
+@pytest.mark.parametrize("key,row", ROWS, ids=[k for k, _ in ROWS])
+def test_library_takes_the_tuned_shape(key, row):
+    method, arith, n, s, nsys = A.parse_batched_key(key)
+    assert slice_shape(n, s, nsys, method, arith) == tuple(row["choice"])
+
+
+@pytest.mark.parametrize("key,row", ROWS, ids=[k for k, _ in ROWS])
+def test_choice_keeps_the_answers_and_is_fastest(key, row):
+    """The choice gave the default shape's bits and is within the tuner's
+    margin of the fastest shape that did; when the default is within that
+    margin it is the choice."""
+    method, arith, n, s, nsys = A.parse_batched_key(key)
+    choice = "%dx%d" % tuple(row["choice"])
+    default = "%dx%d" % A.default_batched_shape(s)
+    assert row["same_bits"][choice] and row["same_bits"][default]
+    same = [k for k, v in row["same_bits"].items() if v]
+    best = min(row["us"][k] for k in same)
+    margin = TABLE["margin"]
+    assert row["us"][choice] <= best * (1 + margin) + 1e-9
+    if row["us"][default] <= best * (1 + margin):
+        assert choice == default
+    if arith == "exact":
+        assert all(row["same_bits"].values())  # exact answers never depend on the shape
+
+
+def test_fast_ptpr_s256_keeps_the_default():
+    """Fast PTPR at s = 256 with 4 x 64 or 8 x 32 slices gives other bits, so
+    its default stays 16 x 16 (config 1 among them)."""
+    rows = [r for k, r in ROWS if k.startswith("ptpr/fast/") and "/s256/" in k]
+    assert rows and all(tuple(r["choice"]) == (16, 16) for r in rows)
+    assert not any(r["same_bits"]["4x64"] or r["same_bits"]["8x32"] for r in rows)
+    assert slice_shape(1024, 256, 4096, "ptpr", "fast") == (16, 16)
+
+
+def _row(method, arith, n, s, nsys):
+    return tuple(TABLE["cases"][A.batched_key(method, arith, n, s, nsys)]["choice"])
+
+
+@pytest.mark.parametrize("nsys,tuned", [(0, 1), (2, 1), (3, 1), (100, 256), (1000, 256), (3000, 4096),
+                                        (100000, 4096)])
+def test_nearest_tuned_batch_size(nsys, tuned):
+    for method in ("ptpr", "tpr"):
+        for arith in ("fast", "exact"):
+            assert slice_shape(2048, 512, nsys, method, arith) == _row(method, arith, 2048, 512, tuned)
+
+
+def test_capped_batch_row_for_the_largest_class():
+    """n = 8192 was tuned with 2048 systems (the 16 Mi-equation cap)."""
+    for method in ("ptpr", "tpr"):
+        assert slice_shape(8192, 512, 4096, method, "exact") == _row(method, "exact", 8192, 512, 2048)
+
+
+@pytest.mark.parametrize("n,s,shape", [(512, 128, (8, 16)), (4096, 1024, (16, 64)), (16, 16, (1, 16)),
+                                       (8192, 256, (16, 16)), (32768, 2048, (32, 64))])
+def test_default_shape_outside_the_table(n, s, shape):
+    for method in ("ptpr", "tpr"):
+        for arith in ("fast", "exact"):
+            assert slice_shape(n, s, 7, method, arith) == shape
+
+
+def test_table_covers_every_tuned_case():
+    keys = set(TABLE["cases"])
+    for method in ("ptpr", "tpr"):
+        for arith in ("fast", "exact"):
+            for s in (256, 512):
+                for m in (1, 2, 4, 8, 16):
+                    n = s * m
+                    for nsys in (1, 256, min(4096, TABLE["eq_cap"] // n)):
+                        assert A.batched_key(method, arith, n, s, nsys) in keys
 
 
 def test_errors_as_the_batched_entry():
@@ -41,3 +100,9 @@ def test_errors_as_the_batched_entry():
     with pytest.raises(_lib.TprError) as e:
         slice_shape(65536, 256)       # 256 slices x 16 lanes: beyond one workgroup
     assert e.value.status == _lib.ERROR_NOT_SUPPORTED
+
+
+def test_table_file_is_json():
+    with open(A.BATCHED_PATH) as f:
+        assert json.load(f)["cases"]
+    assert os.path.dirname(A.BATCHED_PATH).endswith("schedules")

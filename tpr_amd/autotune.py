@@ -14,6 +14,14 @@ This generalises the reference's fixed launch shape for one (n, s)
 choice per size class.
 
     python -m tpr_amd.autotune --out tpr_amd/schedules/uneven_gfx950.json
+
+With --batched it measures the batched entry point instead: for each
+(method, arithmetic, n, s, batch size) every compiled slice shape (E, L)
+with E * L = s that fits a workgroup, whether it gives the same bits as the
+default shape, and its time; the fastest same-bits shape is the choice, and
+the library's built-in rule (tprh_slice_shape) is recorded beside it.
+
+    python -m tpr_amd.autotune --batched --out tpr_amd/schedules/batched_gfx950.json
 """
 import argparse
 import json
@@ -90,6 +98,142 @@ def tune(sizes=None, method="ptpr", eq_target=16 << 20, reps=20, warmup=3, log=N
     return res
 
 
+# (E, L) slice shapes compiled into the batched kernel
+BATCHED_SHAPES = [(1, 2), (1, 4), (1, 8), (1, 16), (2, 16), (4, 16), (8, 16), (16, 16), (4, 64), (8, 32), (8, 64),
+                  (16, 32), (16, 64), (32, 64)]
+BATCHED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules", "batched_gfx950.json")
+
+
+def default_batched_shape(s):
+    """The library's default (E, L) for slice size s: one DPP row of 16 lanes
+    per slice up to s = 256, then 32 or 64 lanes."""
+    if s <= 16:
+        return (1, s)
+    if s <= 256:
+        return (s // 16, 16)
+    return {512: (16, 32), 1024: (16, 64), 2048: (32, 64)}[s]
+
+
+def batched_candidates(n, s):
+    """(E, L) shapes the batched entry point can run for (n, s) in one workgroup."""
+    m = n // s
+    return [(E, L) for E, L in BATCHED_SHAPES if E * L == s and m * L <= 1024]
+
+
+def batched_key(method, arith, n, s, nsys):
+    return "%s/%s/n%d/s%d/nsys%d" % (method, arith, n, s, nsys)
+
+
+def _time_batched(sets, n, s, method, arith, shape, reps, warmup):
+    import torch
+    from .batched import solve_batched
+    for i in range(warmup):
+        t = sets[i % len(sets)]
+        solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method, arith, shape=shape)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(reps):
+        t = sets[i % len(sets)]
+        solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method, arith, shape=shape)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1000.0 / reps
+
+
+def pick(row, default, margin):
+    """The fastest shape whose answers equal the default shape's; the default
+    itself unless another is faster by more than `margin` (a fraction), so
+    that timing noise does not move the choice."""
+    dkey = "%dx%d" % default
+    same = [k for k, v in row["same_bits"].items() if v]
+    best = min(same, key=lambda k: row["us"][k])
+    if dkey in same and row["us"][dkey] <= row["us"][best] * (1.0 + margin):
+        best = dkey
+    E, L = best.split("x")
+    return int(E), int(L)
+
+
+def tune_batched(slice_sizes=(256, 512), slices=(1, 2, 4, 8, 16), batches=(1, 256, 4096), methods=("ptpr", "tpr"),
+                 ariths=("fast", "exact"), max_equations=16 << 20, reps=50, warmup=3, passes=2, margin=0.03,
+                 log=None):
+    """Measure every candidate shape of the batched entry point; returns
+    {"cases": {key: {"us": {"ExL": t}, "same_bits": {"ExL": bool}, "choice": [E, L], "rule": [E, L]}}}.
+    Buffer sets are cycled so that a batch larger than a few MB is read from
+    HBM, not from the last-level cache; "same_bits" compares each shape's
+    answers with the default shape's on the same inputs; "rule" is the shape
+    the library took for these arguments when the table was made."""
+    import torch
+    from .batched import alloc_batch, generate, slice_shape, solve_batched
+    res = {"eq_cap": max_equations, "reps": reps, "passes": passes, "margin": margin, "cases": {}}
+    for s in slice_sizes:
+        for m in slices:
+            n = s * m
+            for nsys in batches:
+                nsys = max(1, min(nsys, max_equations // n))
+                set_bytes = 20 * n * nsys
+                nsets = max(1, min(8, -(-(600 << 20) // set_bytes)))
+                sets = [alloc_batch(nsys, n, with_xt=False) for _ in range(nsets)]
+                for k, t in enumerate(sets):
+                    generate("dominant", 77, k * nsys, nsys, n, t["a"], t["b"], t["c"], t["d"])
+                t0 = sets[0]
+                for method in methods:
+                    for arith in ariths:
+                        cands = batched_candidates(n, s)
+                        rule = slice_shape(n, s, nsys, method, arith)
+                        solve_batched(t0["a"], t0["b"], t0["c"], t0["d"], t0["x"], n, s, method, arith,
+                                      shape=default_batched_shape(s))
+                        ref = t0["x"].clone()
+                        row = {"us": {}, "same_bits": {}}
+                        for shape in cands:
+                            key = "%dx%d" % shape
+                            solve_batched(t0["a"], t0["b"], t0["c"], t0["d"], t0["x"], n, s, method, arith,
+                                          shape=shape)
+                            torch.cuda.synchronize()
+                            row["same_bits"][key] = bool(torch.equal(t0["x"].view(torch.int32),
+                                                                     ref.view(torch.int32)))
+                            row["us"][key] = float("inf")
+                        for _ in range(passes):  # interleaved passes, the best time of each shape
+                            for shape in cands:
+                                key = "%dx%d" % shape
+                                us = _time_batched(sets, n, s, method, arith, shape, reps, warmup)
+                                row["us"][key] = round(min(row["us"][key], us), 2)
+                        row["choice"] = list(pick(row, default_batched_shape(s), margin))
+                        row["rule"] = list(rule)
+                        key = batched_key(method, arith, n, s, nsys)
+                        res["cases"][key] = row
+                        if log:
+                            log("%s rule %dx%d choice %dx%d %s" % (key, rule[0], rule[1], row["choice"][0],
+                                                                 row["choice"][1], row["us"]))
+                del sets
+                torch.cuda.empty_cache()
+    return res
+
+
+HEADER_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "batched_shapes.inc")
+
+
+def parse_batched_key(key):
+    method, arith, n, s, nsys = key.split("/")
+    return method, arith, int(n[1:]), int(s[1:]), int(nsys[4:])
+
+
+def header_text(table):
+    """The rows of the batched table as the C initialiser capi.hip compiles
+    in: {method, arith, s, m, nsys, E, L} (method 0 PTPR / 1 TPR, arith 0
+    fast / 1 exact), in a fixed order."""
+    rows = []
+    for key, r in table["cases"].items():
+        method, arith, n, s, nsys = parse_batched_key(key)
+        rows.append((0 if method == "ptpr" else 1, 0 if arith == "fast" else 1, s, n // s, nsys, r["choice"][0],
+                     r["choice"][1]))
+    lines = ["// Generated by `python -m tpr_amd.autotune --batched` from schedules/batched_gfx950.json;",
+             "// {method (0 PTPR, 1 TPR), arith (0 fast, 1 exact), s, m, nsys, E, L}: the fastest slice shape",
+             "// measured on MI355X whose answers equal the default shape's."]
+    lines += ["{%d, %d, %d, %d, %d, %d, %d}," % row for row in sorted(rows)]
+    return "\n".join(lines) + "\n"
+
+
 def schedule_of(table):
     """n -> (s, E, L) mapping of a tuned table."""
     return {int(n): tuple(r["choice"]) for n, r in table["classes"].items()}
@@ -113,8 +257,19 @@ def main(argv=None):
     ap.add_argument("--min-log2", type=int, default=8)
     ap.add_argument("--max-log2", type=int, default=14)
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--batched", action="store_true", help="tune the batched entry point's slice shapes")
+    ap.add_argument("--header", default=None, help="with --batched: also write the table as capi.hip's C rows")
     args = ap.parse_args(argv)
     t0 = time.time()
+    if args.batched:
+        table = tune_batched(log=lambda m: print(m, flush=True))
+        table["tuned_seconds"] = round(time.time() - t0, 1)
+        if args.out:
+            save(table, args.out)
+        if args.header:
+            with open(args.header, "w") as f:
+                f.write(header_text(table))
+        return
     table = tune([1 << k for k in range(args.min_log2, args.max_log2 + 1)], args.method, reps=args.reps,
                  log=lambda m: print(m, flush=True))
     table["tuned_seconds"] = round(time.time() - t0, 1)

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -42,37 +43,52 @@ tprh_status fail_hip(hipError_t e, const char* what) {
 
 bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 
-// Where a wider slice shape than default_shape()'s measured faster on MI355X
-// (1 to 4096 systems, inputs in HBM), with m = n / s slices per system:
-//   s = 512, 8 x 64 instead of 16 x 32: PTPR up to m = 8 (fast 12-37 %,
-//     exact 14-39 %; fast loses ~3 % on large batches at m = 16, a 1024-lane
-//     workgroup per system), TPR up to m = 4 (fast 3-6 %; within +-3 % at
-//     m = 8, up to 34 % slower at m = 16);
-//   s = 256, 4 x 64 instead of 16 x 16: exact PTPR up to m = 16 (13-47 %);
-//   s = 256, 8 x 32 instead of 16 x 16: exact TPR up to m = 8 in batches of
-//     at most 256 systems (12-26 %; equal or slower from 1024 systems on);
-//     fast PTPR and fast TPR keep 16 x 16 (fast PTPR, 4096 systems of
-//     1024: 21 us against 28 us).
-// The answers are the same bits with either shape in these cases (exact
-// arithmetic always; fast arithmetic for the s = 512 pair), so the choice
-// changes speed only; in exact arithmetic it may depend on the batch size.
-bool wider_shape(int method, int arith, int s, int m, int nsys, int* E, int* L) {
-    if (s == 512 && m <= (method == TPRH_PTPR ? 8 : 4)) {
-        *E = 8;
-        *L = 64;
-        return true;
+// Slice shapes measured on MI355X by the batched tuner (tpr_amd/autotune.py
+// --batched, schedules/batched_gfx950.json): for PTPR and TPR, fast and
+// exact, s = 256 and 512, m = n / s = 1 .. 16 slices per system and batches
+// of 1, 256 and 4096 systems (2048 for n = 8192), every compiled shape with
+// E * L = s that fits a workgroup was timed on inputs in HBM; a row keeps the
+// fastest shape whose answers equal the default shape's on the tuning inputs
+// (the default unless another is faster by more than 3 %).  Exact arithmetic
+// gives the same bits with every shape; in fast arithmetic PTPR at s = 256
+// does not, so its rows keep the default shape.
+struct TunedShape {
+    int method, arith, s, m, nsys, E, L;
+};
+const TunedShape kTunedShapes[] = {
+#include "batched_shapes.inc"
+};
+
+int floor_log2(long long v) {
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        r++;
     }
-    if (s == 256 && method == TPRH_PTPR && arith == TPRH_ARITH_EXACT && m <= 16) {
-        *E = 4;
-        *L = 64;
-        return true;
+    return r;
+}
+
+// The tuned shape for (method, arith, s, m): the row whose batch size is
+// nearest to nsys on a log scale (the smaller on a tie); false when no row
+// covers (method, arith, s, m), which keeps default_shape().
+bool tuned_shape(int method, int arith, int s, int m, int nsys, int* E, int* L) {
+    const int mi = method == TPRH_PTPR ? 0 : 1;
+    const int ai = arith == TPRH_ARITH_EXACT ? 1 : 0;
+    const int lg = floor_log2(nsys > 1 ? nsys : 1);
+    const TunedShape* best = nullptr;
+    int best_d = 0;
+    for (const TunedShape& t : kTunedShapes) {
+        if (t.method != mi || t.arith != ai || t.s != s || t.m != m) continue;
+        const int d = std::abs(floor_log2(t.nsys) - lg);
+        if (!best || d < best_d || (d == best_d && t.nsys < best->nsys)) {
+            best = &t;
+            best_d = d;
+        }
     }
-    if (s == 256 && method == TPRH_TPR && arith == TPRH_ARITH_EXACT && m <= 8 && nsys <= 256) {
-        *E = 8;
-        *L = 32;
-        return true;
-    }
-    return false;
+    if (!best) return false;
+    *E = best->E;
+    *L = best->L;
+    return true;
 }
 
 // Checks (n, s) for PTPR/TPR and picks the slice shape.  The reference
@@ -80,8 +96,8 @@ bool wider_shape(int method, int arith, int s, int m, int nsys, int* E, int* L) 
 // src/pcr.cpp:16) and silently misbehaves otherwise; here they are errors.
 // *split is set when a system needs more lanes than one workgroup has.
 // wide_method >= 0: the caller's variant exists for every compiled shape
-// (fp32 coefficients, no refinement), so the default may take the wider
-// shape wider_shape() names for that method and arithmetic.
+// (fp32 coefficients, no refinement), so the default may take the shape
+// tuned_shape() names for that method and arithmetic.
 tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape, bool allow_split, bool* split,
                      int wide_method = -1, int wide_arith = 0) {
     if (n <= 0 || s <= 0 || nsys < 0) return fail(TPRH_ERROR_INVALID_VALUE, "n, s must be positive and nsys >= 0");
@@ -95,7 +111,7 @@ tprh_status check_ns(int n, int s, int nsys, int* E, int* L, bool explicit_shape
     } else if (!tprh::default_shape(s, E, L)) {
         return fail(TPRH_ERROR_NOT_SUPPORTED, "slice size s above 2048 is not supported");
     } else if (wide_method >= 0) {
-        (void)wider_shape(wide_method, wide_arith, s, m, nsys, E, L);
+        (void)tuned_shape(wide_method, wide_arith, s, m, nsys, E, L);
     }
     *split = (long long)m * *L > 1024;
     if (*split) {
