@@ -125,19 +125,31 @@ def batched_key(method, arith, n, s, nsys):
 
 
 def _time_batched(sets, n, s, method, arith, shape, reps, warmup):
+    """Device time per launch: `reps` launches captured in one HIP graph and
+    replayed, so that small batches are timed on the GPU, not by the rate at
+    which the host issues launches."""
     import torch
     from .batched import solve_batched
-    for i in range(warmup):
+
+    def launch(i):
         t = sets[i % len(sets)]
         solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method, arith, shape=shape)
+
+    for i in range(warmup):
+        launch(i)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for i in range(reps):
+            launch(i)
+    g.replay()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
-    for i in range(reps):
-        t = sets[i % len(sets)]
-        solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method, arith, shape=shape)
+    g.replay()
     e1.record()
     torch.cuda.synchronize()
+    del g
     return e0.elapsed_time(e1) * 1000.0 / reps
 
 
@@ -159,8 +171,9 @@ def tune_batched(slice_sizes=(256, 512), slices=(1, 2, 4, 8, 16), batches=(1, 25
                  log=None):
     """Measure every candidate shape of the batched entry point; returns
     {"cases": {key: {"us": {"ExL": t}, "same_bits": {"ExL": bool}, "choice": [E, L], "rule": [E, L]}}}.
-    Buffer sets are cycled so that a batch larger than a few MB is read from
-    HBM, not from the last-level cache; "same_bits" compares each shape's
+    Launches are timed from a replayed HIP graph; buffer sets are cycled so
+    that a batch larger than a few MB is read from HBM, not from the
+    last-level cache; "same_bits" compares each shape's
     answers with the default shape's on the same inputs; "rule" is the shape
     the library took for these arguments when the table was made."""
     import torch
