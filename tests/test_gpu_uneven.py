@@ -56,6 +56,8 @@ SCHEDULES = {
     "default": None,
     "shapes": {256: (256, 8, 32), 512: (512, 8, 64), 1024: (256, 4, 64), 2048: (128, 8, 16),
                4096: (256, 16, 16), 8192: (512, 8, 64), 64: (16, 1, 16), 128: (32, 2, 16)},
+    # s alone: exact arithmetic takes a shape of the kernel's menu for s = 512
+    "s512": {1024: (512,), 2048: (512,), 4096: (512,), 8192: (512,)},
 }
 
 

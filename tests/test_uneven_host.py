@@ -10,12 +10,12 @@ from tpr_amd import _lib, autotune
 from tpr_amd.uneven import schedule_array
 
 
-def _create(sizes, schedule=None, method=_lib.PTPR_M):
+def _create(sizes, schedule=None, method=_lib.PTPR_M, arith=0):
     L = _lib.lib()
     h = ctypes.c_void_p()
     sz = np.asarray(sizes, np.int32)
     tab = schedule_array(schedule) if schedule else None
-    st = L.tprh_uneven_plan_create(ctypes.byref(h), method, 0, sz.ctypes.data_as(ctypes.c_void_p), sz.size,
+    st = L.tprh_uneven_plan_create(ctypes.byref(h), method, arith, sz.ctypes.data_as(ctypes.c_void_p), sz.size,
                                    tab.ctypes.data_as(ctypes.c_void_p) if tab is not None else None)
     return st, h
 
@@ -37,6 +37,21 @@ def test_bad_method_and_schedule_rejected():
     assert _create([4096], {4096: (256, 8, 16)})[0] == _lib.ERROR_INVALID_VALUE
     # too many lanes per system: 16384 / 128 slices * 16 lanes > 1024
     assert _create([16384], {16384: (128, 8, 16)})[0] == _lib.ERROR_NOT_SUPPORTED
+
+
+@pytest.mark.parametrize("method", [_lib.PTPR_M, _lib.TPR_M])
+@pytest.mark.parametrize("n", [1024, 4096, 8192])
+def test_exact_classes_at_s512_take_a_menu_shape(method, n):
+    """A schedule row giving only s = 512: the default 16 x 32 slices have no
+    path in the persistent kernel, so fast arithmetic refuses the class;
+    exact arithmetic (the same bits with any shape) takes a shape of the
+    menu and the plan passes validation (here it then fails at the device
+    allocation, without a GPU)."""
+    st, h = _create([n, 256], {n: (512, 0, 0)}, method=method, arith=1)
+    assert st in (_lib.SUCCESS, _lib.ERROR_HIP), st
+    if st == _lib.SUCCESS:
+        _lib.lib().tprh_uneven_plan_destroy(h)
+    assert _create([n, 256], {n: (512, 0, 0)}, method=method, arith=0)[0] == _lib.ERROR_NOT_SUPPORTED
 
 
 def test_schedule_array_layout():

@@ -380,11 +380,23 @@ tprh_status class_shape(int n, const int* schedule, int* s_out, int* E_out, int*
         expl = E != 0 || L != 0;
     }
     // exact arithmetic gives the same bits with any shape, so a class without
-    // a schedule row may take the wider shapes the batched entry points take
-    // (the persistent kernels carry them: 8 x 32, 4 x 64, 8 x 64)
+    // an explicit shape takes the batched entry's tuned shape for one system
+    // and, where that one has no path in the persistent kernels, the first
+    // shape of their menu that fits
     tprh_status st = check_ns(n, s, 1, &E, &L, expl, allow_split, split,
                               arith == TPRH_ARITH_EXACT ? method : -1, arith);
     if (st != TPRH_SUCCESS) return st;
+    if (arith == TPRH_ARITH_EXACT && !expl && !*split && uneven_shape_id(E, L) < 0) {
+        bool found = false;
+#define TPRH_MENU_TRY(ID, EE, LL)                                                 \
+    if (!found && (EE) * (LL) == s && (long long)(n / s) * (LL) <= 1024) { \
+        E = EE;                                                                \
+        L = LL;                                                                \
+        found = true;                                                          \
+    }
+        TPRH_UNEVEN_SHAPES(TPRH_MENU_TRY)
+#undef TPRH_MENU_TRY
+    }
     *s_out = s;
     *E_out = E;
     *L_out = L;
