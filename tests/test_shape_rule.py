@@ -47,12 +47,13 @@ def test_choice_keeps_the_answers_and_is_fastest(key, row):
         assert all(row["same_bits"].values())  # exact answers never depend on the shape
 
 
-def test_fast_ptpr_s256_keeps_the_default():
-    """Fast PTPR at s = 256 with 4 x 64 or 8 x 32 slices gives other bits, so
-    its default stays 16 x 16 (config 1 among them)."""
-    rows = [r for k, r in ROWS if k.startswith("ptpr/fast/") and "/s256/" in k]
-    assert rows and all(tuple(r["choice"]) == (16, 16) for r in rows)
-    assert not any(r["same_bits"]["4x64"] or r["same_bits"]["8x32"] for r in rows)
+def test_every_shape_gives_the_same_bits():
+    """Every candidate gave the default shape's answers on the tuning inputs,
+    in both arithmetic modes (the fast stage 1 has one operation order for
+    every shape), so no row's choice is limited by its answers; config 1
+    (fast PTPR, 4096 x 1024, s = 256) keeps 16 x 16, its fastest shape."""
+    for key, row in ROWS:
+        assert all(row["same_bits"].values()), key
     assert slice_shape(1024, 256, 4096, "ptpr", "fast") == (16, 16)
 
 

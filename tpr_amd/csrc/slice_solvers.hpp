@@ -255,9 +255,12 @@ __device__ __forceinline__ void ptpr_stage1_fast16(float (&Ap)[E], float (&Cp)[E
                 den = __builtin_fmaf(-Ap[j + U], ck, __builtin_fmaf(-Cp[j - U], ak, 1.0f));
                 rn = __builtin_fmaf(R[j + U], ck, __builtin_fmaf(R[j - U], ak, R[j]));
             } else if (lx) {
-                den = __builtin_fmaf(-Ap[j + U], ck, 1.0f);
-                rn = __builtin_fmaf(R[j + U], ck, R[j]);
+                // left partner first, as in the other branches
+                den = 1.0f;
+                rn = R[j];
                 dpp_den_rn_up<1>(den, rn, Cp[lx ? E - U + j : 0], R[lx ? E - U + j : 0], ak);
+                den = __builtin_fmaf(-Ap[j + U], ck, den);
+                rn = __builtin_fmaf(R[j + U], ck, rn);
             } else {
                 den = __builtin_fmaf(-Cp[j - U], ak, 1.0f);
                 rn = __builtin_fmaf(R[j - U], ak, R[j]);

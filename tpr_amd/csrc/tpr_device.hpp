@@ -177,10 +177,14 @@ __device__ __forceinline__ Eq pcr_update(Eq k, Eq kl, Eq kr, bool hasL, bool has
         rn = hasR ? (rn - kr.r * k.c) : rn;
         o.r = inv * rn;
     } else {
+        // the operation order of the fused DPP form (slice_solvers.hpp,
+        // ptpr_stage1_fast16): left partner first, inv times the row's own
+        // coefficient before the partner's, so that every slice shape gives
+        // the same bits
         float den = __builtin_fmaf(-kr.a, k.c, __builtin_fmaf(-kl.c, k.a, 1.0f));
         float inv = fast_rcp(den);
-        o.a = (-inv * kl.a) * k.a;
-        o.c = (-inv * kr.c) * k.c;
+        o.a = (-inv * k.a) * kl.a;
+        o.c = (-inv * k.c) * kr.c;
         float rn = __builtin_fmaf(-kr.r, k.c, __builtin_fmaf(-kl.r, k.a, k.r));
         o.r = inv * rn;
     }
