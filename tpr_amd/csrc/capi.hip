@@ -49,9 +49,9 @@ bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 // of 1, 256 and 4096 systems (2048 for n = 8192), every compiled shape with
 // E * L = s that fits a workgroup was timed on inputs in HBM; a row keeps the
 // fastest shape whose answers equal the default shape's on the tuning inputs
-// (the default unless another is faster by more than 3 %).  Exact arithmetic
-// gives the same bits with every shape; in fast arithmetic PTPR at s = 256
-// does not, so its rows keep the default shape.
+// (the default unless another is faster by more than 3 %).  Both arithmetic
+// modes give the same bits with every shape (one operation order per mode),
+// so the choice changes the speed only.
 struct TunedShape {
     int method, arith, s, m, nsys, E, L;
 };
