@@ -105,8 +105,9 @@ class Server:
         return out.view(inf["nrhs"], inf["n"])
 
     def poll(self, max_ids=4096, timeout=None):
-        """Ids of requests completed since the last poll; with a timeout (s),
-        sleeps until at least one has completed or the timeout passes."""
+        """Ids of requests completed since the last poll, oldest first
+        (requests already released are left out); with a timeout (s), sleeps
+        until at least one has completed or the timeout passes."""
         buf = (ctypes.c_longlong * max_ids)()
         cnt = ctypes.c_int()
         if timeout is None:
