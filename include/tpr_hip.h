@@ -91,7 +91,13 @@ tprh_status tprh_cr_cu(const float *a, const float *c, const float *rhs, float *
  * events around the launch (src/ptpr.cu:574-587, src/pm.cuh:30-59): when
  * enabled, each such call records HIP events around its kernel launch and
  * tprh_last_kernel_ms() returns that duration for the calling thread's last
- * call (-1 when timing was off). */
+ * call (-1 when timing was off).  Systems of up to 65536 equations on the
+ * one-workgroup path are solved straight from mapped pinned host memory (no
+ * copy engine: the kernel reads a, c, rhs and writes x over the host link),
+ * so for them this duration includes those transfers; larger systems are
+ * copied to device memory first and the duration is the kernels' alone.
+ * The host-pointer entry points are thread-safe; calls from different
+ * threads run concurrently, each on a non-blocking stream of its own. */
 tprh_status tprh_set_kernel_timing(int enable);
 float tprh_last_kernel_ms(void);
 

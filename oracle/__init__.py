@@ -182,18 +182,21 @@ def _need_ref():
     return lib
 
 
-def ref_ptpr(a, c, rhs, s):
+def ref_ptpr(a, c, rhs, s, with_flops=False):
+    """The reference PTPR class; with_flops: also what its solve() returned
+    (the analytic flop count, src/ptpr.cpp:103-106)."""
     a, c, rhs = _c32(a), _c32(c), _c32(rhs)
     x = np.empty_like(a)
-    _need_ref().ref_ptpr_solve(a, c, rhs, x, a.size, s)
-    return x
+    f = _need_ref().ref_ptpr_solve(a, c, rhs, x, a.size, s)
+    return (x, f) if with_flops else x
 
 
-def ref_tpr(a, c, rhs, s):
+def ref_tpr(a, c, rhs, s, with_flops=False):
+    """The reference TPR class; with_flops as ref_ptpr (src/tpr.cpp:95-98)."""
     a, c, rhs = _c32(a), _c32(c), _c32(rhs)
     x = np.empty_like(a)
-    _need_ref().ref_tpr_solve(a, c, rhs, x, a.size, s)
-    return x
+    f = _need_ref().ref_tpr_solve(a, c, rhs, x, a.size, s)
+    return (x, f) if with_flops else x
 
 
 def ref_pcr(a, c, rhs):

@@ -14,9 +14,14 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def golden():
+    import json
+
     import numpy as np
     d = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(d, "ref_flops.json")) as f:
+        flops = json.load(f)
     return {
+        "flops": flops,
         "ans1024": np.load(os.path.join(d, "ans1024.npy")),
         "nstab": dict(np.load(os.path.join(d, "nstab.npz"))),
         "ref": dict(np.load(os.path.join(d, "ref_vectors.npz"))),

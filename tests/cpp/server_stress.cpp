@@ -402,7 +402,18 @@ static void client(tprh_server* sv, int tid, int iters, uint64_t seed, std::atom
         }
         for (Req& r : reqs) {
             if (r.id <= 0) continue;
-            const tprh_status ws = tprh_server_wait(sv, r.id, 120.0);
+            // every other request is waited for in short slices, so that
+            // some waits end at their deadline while the request completes
+            // or fails: such a wait must report the request's own outcome
+            // once it has one, never a timeout for a finished request
+            tprh_status ws;
+            if (rng() % 2) {
+                do {
+                    ws = tprh_server_wait(sv, r.id, 1e-5 * (double)(rng() % 50));
+                } while (ws == TPRH_ERROR_TIMEOUT);
+            } else {
+                ws = tprh_server_wait(sv, r.id, 120.0);
+            }
             tprh_request_info inf;
             check_ok(tprh_server_request_info(sv, r.id, &inf), "info");
             if (ws == TPRH_ERROR_HIP && g_fail_every > 0) {  // its batch's launch was made to fail

@@ -13,11 +13,16 @@ reference is present and oracle/_ref was built by `make -C oracle ref`).
                    PCR / CR, the 4-argument normalisation, and systems that
                    force the s1 = -1 and key = 0 branches (src/ptpr.cpp:196-200,
                    :276-279)
+  ref_flops.json   what the reference classes' solve() returns (the analytic
+                   flop count, src/ptpr.cpp:103-106, src/tpr.cpp:95-98) for
+                   every (n, s) of FLOP_CASES, taken from the return value of
+                   a real solve
 
-    python tests/golden/make_golden.py [--ref /root/reference]
+    python tests/golden/make_golden.py [--ref /root/reference] [--only flops]
 """
 import argparse
 import glob
+import json
 import os
 import re
 import sys
@@ -33,6 +38,23 @@ import oracle as O  # noqa: E402
 PTPR_CASES = [(16, 2), (16, 4), (64, 8), (64, 64), (256, 16), (512, 32), (1024, 256), (1024, 4),
               (2048, 512), (2048, 64), (4096, 1024), (128, 128)]
 PCR_CR_N = [2, 16, 256, 1024]
+# every solve shape of the parity cases, the benchmark configurations
+# (2048/512, 1024/256, 65536/256) and the uneven size classes at s = 256
+FLOP_CASES = sorted(set(PTPR_CASES + [(2048, 512), (1024, 256), (65536, 256), (256, 256), (512, 256),
+                                      (2048, 256), (4096, 256), (8192, 256), (16384, 256), (32768, 256),
+                                      (4096, 2), (8192, 8), (1024, 1024), (32768, 1024)]))
+
+
+def write_flops():
+    out = {"ptpr": {}, "tpr": {}}
+    for (n, s) in FLOP_CASES:
+        a, b, c, d, _ = O.gen_system(1, 3, n)
+        na, nc, nd = O.ref_normalize(a, b, c, d, s)
+        out["ptpr"]["%d,%d" % (n, s)] = int(O.ref_ptpr(na, nc, nd, s, with_flops=True)[1])
+        out["tpr"]["%d,%d" % (n, s)] = int(O.ref_tpr(na, nc, nd, s, with_flops=True)[1])
+    with open(os.path.join(HERE, "ref_flops.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
 
 
 def read_ans1024(ref):
@@ -76,9 +98,14 @@ def branch_system(n, s, seed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", choices=["flops"], help="write only this fixture")
     args = ap.parse_args()
     if not O.ref_available():
         sys.exit("oracle/_ref/libtprref.so is not built: run `make -C oracle ref`")
+    write_flops()
+    if args.only == "flops":
+        print("wrote", os.path.join(HERE, "ref_flops.json"))
+        return
 
     np.save(os.path.join(HERE, "ans1024.npy"), read_ans1024(args.ref))
 

@@ -279,6 +279,39 @@ def test_cu_dropins_known_answer(golden):
     assert bits_equal(x, O.pcr(a, c, d))
 
 
+def test_cu_dropins_from_many_threads():
+    """The host-pointer entry points from 6 threads at once, each with its
+    own systems and shapes (one-workgroup sizes, solved from mapped pinned
+    memory, and a multi-workgroup size, copied): every answer is the CPU
+    class's, bit for bit."""
+    import threading
+    cases = [(1024, 256, "ptpr"), (2048, 512, "tpr"), (4096, 64, "ptpr"), (65536, 256, "ptpr"),
+             (512, 16, "tpr"), (32768, 2048, "ptpr")]
+    errors = []
+
+    def worker(k, n, s, method):
+        try:
+            fn = tpr_amd.ptpr_cu if method == "ptpr" else tpr_amd.tpr_cu
+            ref_fn = O.ptpr if method == "ptpr" else O.tpr
+            for rep in range(8):
+                a, b, c, d, _ = O.gen_system(50 + k, rep, n)
+                na, nc, nd = O.normalize(a, b, c, d)
+                x = np.full(n, np.nan, np.float32)
+                fn(na, nc, nd, x, n, s)
+                if not bits_equal(x, ref_fn(na, nc, nd, s)):
+                    errors.append((k, n, s, method, rep))
+        except Exception as e:  # reported below, in the main thread
+            errors.append((k, repr(e)))
+
+    say("cu dropins from threads", len(cases))
+    th = [threading.Thread(target=worker, args=(k,) + cs) for k, cs in enumerate(cases)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+
+
 @pytest.mark.parametrize("n", [2, 64, 1024, 4096])
 def test_pcr_cr_match_oracle(n):
     say("pcr/cr", n)

@@ -238,3 +238,29 @@ def test_shared_matrix_fp16_and_refinement():
         sl = slice(j * n, (j + 1) * n)
         ref = oracle_refine("ptpr", a, b, c, rhs[sl], x0[sl], n, s)
         assert bits_equal(x1[j], ref), j
+
+
+def test_solve_ex_side_stream_keeps_its_workspace():
+    """solve_ex allocates the split path's workspace itself on the current
+    stream and returns while its kernels may still run on `stream`: the
+    caching allocator must not hand those bytes to an allocation made on
+    the current stream at once.  Answers equal a same-stream solve."""
+    n, s, nsys, chunk = 1 << 18, 256, 4, 1
+    say("side stream workspace", n, s, nsys, chunk)
+    a, b, c, d, _ = O.gen_batch(38, 0, nsys, n, O.GEN_WEAK)
+    da, db, dc, dd = dev(a), dev(b), dev(c), dev(d)
+    x_ref = torch.empty(nsys * n, device="cuda")
+    B.solve_ex(da, db, dc, dd, x_ref, n, s, arith="exact", chunk=chunk)
+    torch.cuda.synchronize()
+    need = B.workspace_bytes(B.make_desc(da, db, dc, dd, x_ref, n, s, arith="exact", chunk=chunk))
+    assert need > 0
+    side = torch.cuda.Stream()
+    x = torch.zeros(nsys * n, device="cuda")
+    side.wait_stream(torch.cuda.current_stream())
+    for _ in range(3):
+        B.solve_ex(da, db, dc, dd, x, n, s, arith="exact", chunk=chunk, stream=side)
+        # allocations of the workspace's size, filled at once on the current stream
+        junk = [torch.full((need // 4 + 1,), float("nan"), device="cuda") for _ in range(4)]
+        del junk
+    torch.cuda.synchronize()
+    assert bits_equal(x.cpu().numpy(), x_ref.cpu().numpy())

@@ -9,7 +9,12 @@ against it.
   TPRTest/PTPRTest only assert the last s of their loop);
 * against closed forms and independent solvers: diagonal systems (x = d/b
   exactly), FP64 Thomas (itself checked against scipy.linalg.solve_banded),
-  dense numpy.linalg.solve for small n.
+  dense numpy.linalg.solve for small n;
+* the analytic flop counts against what the reference classes' solve()
+  returned (tests/golden/ref_flops.json, and live);
+* the seeded generator against SplitMix64's published output (its mixing
+  function and Weyl sequence) and against properties of its right-hand-side
+  stream: range, grid, index layout, independence.
 """
 import numpy as np
 import pytest
@@ -256,3 +261,159 @@ def test_norm_formula():
     diff = (x - t).astype(np.float32)
     exp = np.sqrt(np.sum((diff * diff).astype(np.float64))) / np.sqrt(np.sum((t * t).astype(np.float64)))
     assert abs(O.norm(x, t) - exp) <= 1e-12 * exp
+
+
+# ---------------------------------------------------------------------------
+# analytic flop counts: what the reference classes' solve() returns
+
+def test_flops_match_reference_fixture(golden):
+    """orc_ptpr_flops / orc_tpr_flops against the return value of the
+    reference's own solve() (src/ptpr.cpp:103-106, src/tpr.cpp:95-98), over
+    slice sizes 2 .. 1024 and 1 .. 256 slices, so that a swapped log2(s) /
+    log2(m) or a dropped stage term changes some case."""
+    fl = golden["flops"]
+    assert len(fl["ptpr"]) >= 20 and set(fl["ptpr"]) == set(fl["tpr"])
+    for key, want in fl["ptpr"].items():
+        n, s = map(int, key.split(","))
+        assert O.ptpr_flops(n, s) == want, ("ptpr", n, s)
+    for key, want in fl["tpr"].items():
+        n, s = map(int, key.split(","))
+        assert O.tpr_flops(n, s) == want, ("tpr", n, s)
+
+
+def test_flops_fixture_holds_the_documented_count(golden):
+    # the count the reference's documented run (tpr_pm 2048 512 ... PTPR)
+    # reports per solve, and config 1's
+    assert golden["flops"]["ptpr"]["2048,512"] == 266419
+    assert golden["flops"]["ptpr"]["1024,256"] == 118963
+
+
+@needs_ref
+@pytest.mark.parametrize("n,s", [(64, 2), (1024, 256), (2048, 512), (8192, 32), (65536, 256)])
+def test_flops_match_live_reference(n, s):
+    a, b, c, d, _ = O.gen_system(2, 1, n)
+    na, nc, nd = O.normalize(a, b, c, d)
+    x, f = O.ref_ptpr(na, nc, nd, s, with_flops=True)
+    assert f == O.ptpr_flops(n, s)
+    assert bits_equal(x, O.ptpr(na, nc, nd, s))
+    x, f = O.ref_tpr(na, nc, nd, s, with_flops=True)
+    assert f == O.tpr_flops(n, s)
+    assert bits_equal(x, O.tpr(na, nc, nd, s))
+
+
+# ---------------------------------------------------------------------------
+# the generator: SplitMix64 and the right-hand-side stream
+
+_M64 = (1 << 64) - 1
+_GAMMA = 0x9E3779B97F4A7C15
+
+
+def _splitmix64(state, count):
+    """SplitMix64 (Steele, Lea, Flood 2014; Vigna's splitmix64.c): the
+    state advances by the golden gamma and each output is the state
+    through the variant-13 finaliser."""
+    out = []
+    for _ in range(count):
+        state = (state + _GAMMA) & _M64
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def test_splitmix64_published_vectors():
+    # the first outputs of SplitMix64 seeded with 0, as published with the
+    # reference implementation
+    assert _splitmix64(0, 5) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F,
+                                 0xF88BB8A8724C81EC, 0x1B39896A51A8749B]
+
+
+@pytest.mark.parametrize("seed,sys", [(0, 0), (1, 7), (12345, 3)])
+def test_gen_rhs_is_splitmix64(seed, sys):
+    """Right-hand side j of system `sys`: element i is the top 24 bits k of
+    output 8(j n + i) + 6 of a SplitMix64 stream whose state is output
+    sys + 1 of a SplitMix64 stream seeded with `seed`, mapped to
+    (k - 2^23) 2^-23."""
+    n, j = 8, 3
+    key = _splitmix64(seed, sys + 1)[-1]
+    outs = _splitmix64(key, 8 * (j * n + n - 1) + 6)
+    exp = np.array([((outs[8 * (j * n + i) + 5] >> 40) - (1 << 23)) / float(1 << 23) for i in range(n)],
+                   dtype=np.float32)
+    assert bits_equal(O.gen_rhs(seed, sys, j, n), exp)
+
+
+def test_gen_rhs_range_and_grid():
+    d = O.gen_rhs(9, 4, 0, 1 << 16)
+    assert d.dtype == np.float32
+    assert np.all(d >= -1.0) and np.all(d < 1.0)
+    k = d.astype(np.float64) * 2 ** 23
+    assert np.all(k == np.round(k))  # multiples of 2^-23: exact in fp32
+    # uniform: mean 0 and variance 1/3 within 5 standard errors
+    assert abs(d.mean()) < 5 * np.sqrt(1 / 3 / d.size)
+    assert abs(d.var() - 1 / 3) < 5 * np.sqrt(4 / 45 / d.size)
+    # about as many distinct values as a 2^24-point grid gives
+    assert np.unique(d).size > 0.99 * d.size
+
+
+def test_gen_rhs_index_layout():
+    """Right-hand side j of length n covers stream indices [j n, (j+1) n):
+    splitting it in halves gives right-hand sides 2j and 2j+1 of length n/2."""
+    for j in (0, 1, 5):
+        full = O.gen_rhs(3, 11, j, 64)
+        halves = np.concatenate([O.gen_rhs(3, 11, 2 * j, 32), O.gen_rhs(3, 11, 2 * j + 1, 32)])
+        assert bits_equal(full, halves), j
+
+
+def test_gen_rhs_independent_of_order_and_other_systems():
+    n = 1024
+    fwd = [O.gen_rhs(5, 2, j, n) for j in range(6)]
+    rev = [O.gen_rhs(5, 2, j, n) for j in reversed(range(6))][::-1]
+    for x, y in zip(fwd, rev):
+        assert bits_equal(x, y)
+    # other systems, seeds and right-hand sides draw other values
+    others = [O.gen_rhs(5, 3, 0, n), O.gen_rhs(6, 2, 0, n), fwd[1]]
+    for o in others:
+        assert np.mean(o == fwd[0]) < 0.01
+    # and none of them repeats a stream of the system generator
+    a, b, c, d, xt = O.gen_system(5, 2, n)
+    for v in (a, c, xt, d):
+        assert np.mean(v == fwd[0]) < 0.01
+    # successive right-hand sides are uncorrelated
+    r = np.corrcoef(fwd[0], fwd[1])[0, 1]
+    assert abs(r) < 5 / np.sqrt(n)
+
+
+def test_thomas_batch_equals_per_system():
+    n, nsys = 500, 7
+    a, b, c, d, _ = O.gen_batch(8, 2, nsys, n, O.GEN_WEAK)
+    xb = O.thomas_f64_batch(a, b, c, d, n)
+    for k in range(nsys):
+        sl = slice(k * n, (k + 1) * n)
+        assert np.array_equal(xb[sl], O.thomas_f64(a[sl], b[sl], c[sl], d[sl])), k
+
+
+@pytest.mark.parametrize("kind", [O.GEN_DOMINANT, O.GEN_WEAK])
+def test_gen_system_streams_are_splitmix64(kind):
+    """a, c, the diagonal's U[0,1) and x_t are streams 0, 1, 2, 3 of the
+    system's SplitMix64 sequence (element i: output 8 i + stream + 1); the
+    diagonal is b = (|a| + |c|) + extra in fp32, d = A x_t summed in FP64."""
+    seed, sys, n = 77, 5, 12
+    key = _splitmix64(seed, sys + 1)[-1]
+    outs = _splitmix64(key, 8 * n)
+    top = lambda st, i: outs[8 * i + st] >> 40  # noqa: E731
+    pm1 = lambda k: np.float32((k - (1 << 23)) / float(1 << 23))  # noqa: E731
+    a = np.array([pm1(top(0, i)) for i in range(n)], np.float32)
+    c = np.array([pm1(top(1, i)) for i in range(n)], np.float32)
+    u = np.array([np.float32(top(2, i) / float(1 << 24)) for i in range(n)], np.float32)
+    xt = np.array([pm1(top(3, i)) for i in range(n)], np.float32)
+    a[0] = 0
+    c[-1] = 0
+    extra = (np.float32(1) + u) if kind == O.GEN_DOMINANT else (np.float32(1e-4) + u * np.float32(0.0099))
+    b = (np.abs(a) + np.abs(c)) + extra.astype(np.float32)
+    d = b.astype(np.float64) * xt
+    d[1:] = a[1:].astype(np.float64) * xt[:-1] + d[1:]
+    d[:-1] = d[:-1] + c[:-1].astype(np.float64) * xt[1:]
+    ga, gb, gc, gd, gx = O.gen_system(seed, sys, n, kind)
+    for got, exp in ((ga, a), (gc, c), (gb, b), (gx, xt), (gd, d.astype(np.float32))):
+        assert bits_equal(got, exp)

@@ -108,14 +108,20 @@ def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sha
     import torch
     desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared)
     need = workspace_bytes(desc)
+    own = None
     if need:
         if workspace is None:
-            workspace = torch.empty(need, dtype=torch.uint8, device=d.device)
+            workspace = own = torch.empty(need, dtype=torch.uint8, device=d.device)
         if workspace.numel() * workspace.element_size() < need:
             raise ValueError("workspace smaller than %d bytes" % need)
         desc.workspace = workspace.data_ptr()
         desc.workspace_bytes = workspace.numel() * workspace.element_size()
     _lib.call("tprh_solve_ex", ctypes.byref(desc), _stream(stream))
+    if own is not None and stream is not None:
+        # the workspace was allocated on the current stream and is freed when
+        # this function returns, while the kernels on `stream` may still use
+        # it: tell the caching allocator to wait for `stream` before reuse
+        own.record_stream(stream)
     return x
 
 

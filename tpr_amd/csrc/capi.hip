@@ -228,35 +228,95 @@ tprh_status batched(tprh_method method, tprh_arith arith, const float* a, const 
     return solve_desc(desc, (hipStream_t)stream);
 }
 
-// Buffers for the host-pointer entry points, one set per device, grown on
-// demand and kept for the life of the process (the reference allocates and
-// frees five buffers per call, src/ptpr.cu:554-598): device arrays
-// a | c | rhs | diag | x | workspace, and a pinned host copy of the first
-// five, through which a call moves its inputs in one transfer and its
-// answer in another.
-struct HostPathBuffer {
-    float* dev = nullptr;
-    size_t cap = 0;  // floats
-    float* pinned = nullptr;
-    size_t pcap = 0;  // floats
-};
-struct HostPathBuffers {
-    std::mutex mu;
-    std::map<int, HostPathBuffer> per_device;
-};
-HostPathBuffers g_host_bufs;
-
-// Timing events of one host-path call, destroyed on every exit path.
-struct EventPair {
+// State of one caller of the host-pointer entry points on one device, kept
+// for the life of the process (the reference allocates and frees five
+// buffers per call, src/ptpr.cu:554-598): its own non-blocking stream (so a
+// call never waits for other work on the device, such as torch's default
+// stream), two timing events, a mapped pinned host buffer a | c | rhs | diag
+// | x, and device memory for the paths that copy.  Contexts live in a pool:
+// a call takes one, uses it without any lock, and gives it back, so calls
+// from different threads run side by side.
+struct HostCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    ~EventPair() {
-        for (hipEvent_t e : ev)
-            if (e) (void)hipEventDestroy(e);
+    float* pinned = nullptr;    // host address
+    float* pinned_d = nullptr;  // the same memory as the device sees it
+    size_t pcap = 0;            // floats
+    float* dev = nullptr;
+    size_t cap = 0;             // floats
+    bool dirty = false;         // a failed call may have left work queued
+};
+
+struct HostCtxPool {
+    std::mutex mu;
+    std::vector<HostCtx*> idle;
+};
+HostCtxPool g_host_ctx;
+
+// Takes an idle context of `device` from the pool (or a new one) and returns
+// it on destruction.
+struct HostCtxLease {
+    HostCtx* ctx = nullptr;
+    explicit HostCtxLease(int device) {
+        std::lock_guard<std::mutex> g(g_host_ctx.mu);
+        for (size_t i = 0; i < g_host_ctx.idle.size(); i++) {
+            if (g_host_ctx.idle[i]->device == device) {
+                ctx = g_host_ctx.idle[i];
+                g_host_ctx.idle.erase(g_host_ctx.idle.begin() + (long)i);
+                return;
+            }
+        }
+        ctx = new (std::nothrow) HostCtx();
+        if (ctx) ctx->device = device;
+    }
+    ~HostCtxLease() {
+        if (!ctx) return;
+        std::lock_guard<std::mutex> g(g_host_ctx.mu);
+        g_host_ctx.idle.push_back(ctx);
     }
 };
 
+tprh_status host_ctx_ready(HostCtx& cx, size_t pinned_floats, size_t dev_floats) {
+    if (!cx.stream) {
+        TPRH_HIP(hipStreamCreateWithFlags(&cx.stream, hipStreamNonBlocking));
+        TPRH_HIP(hipEventCreate(&cx.ev[0]));
+        TPRH_HIP(hipEventCreate(&cx.ev[1]));
+    }
+    if (cx.dirty) {
+        TPRH_HIP(hipStreamSynchronize(cx.stream));
+        cx.dirty = false;
+    }
+    if (cx.pcap < pinned_floats) {
+        if (cx.pinned) (void)hipHostFree(cx.pinned);
+        cx.pinned = cx.pinned_d = nullptr;
+        cx.pcap = 0;
+        TPRH_HIP(hipHostMalloc(reinterpret_cast<void**>(&cx.pinned), pinned_floats * sizeof(float),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+        cx.pcap = pinned_floats;
+        TPRH_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&cx.pinned_d), cx.pinned, 0));
+    }
+    if (cx.cap < dev_floats) {
+        if (cx.dev) (void)hipFree(cx.dev);
+        cx.dev = nullptr;
+        cx.cap = 0;
+        TPRH_HIP(hipMalloc(&cx.dev, dev_floats * sizeof(float)));
+        cx.cap = dev_floats;
+    }
+    return TPRH_SUCCESS;
+}
+
+// Systems up to this many equations are solved straight from the mapped
+// pinned buffer: the kernel reads a, c, rhs (and diag) over the host link
+// and writes x back the same way, so a call is one launch and one stream
+// synchronisation, with no copy engine in between.  Larger systems (and the
+// multi-workgroup path, whose kernels read the inputs twice) are copied to
+// device memory first.
+constexpr int kZeroCopyMaxN = 1 << 16;
+
 tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a, const float* diag,
                             const float* c, const float* rhs, float* x, int n, int s) {
+    g_last_kernel_ms = -1.0f;
     if (!a || !c || !rhs || !x) return fail(TPRH_ERROR_INVALID_VALUE, "null array pointer");
     if (n <= 0) return fail(TPRH_ERROR_INVALID_VALUE, "n must be positive");
     tprh_solve_desc desc{};
@@ -278,46 +338,31 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     }
     int device = 0;
     TPRH_HIP(hipGetDevice(&device));
-    std::lock_guard<std::mutex> lock(g_host_bufs.mu);
-    HostPathBuffer& buf = g_host_bufs.per_device[device];
-    const size_t need = 5 * (size_t)n + (pl.ws + sizeof(float) - 1) / sizeof(float);
-    if (buf.cap < need) {
-        if (buf.dev) (void)hipFree(buf.dev);
-        buf.dev = nullptr;
-        buf.cap = 0;
-        TPRH_HIP(hipMalloc(&buf.dev, need * sizeof(float)));
-        buf.cap = need;
-    }
-    if (buf.pcap < 5 * (size_t)n) {
-        if (buf.pinned) (void)hipHostFree(buf.pinned);
-        buf.pinned = nullptr;
-        buf.pcap = 0;
-        TPRH_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf.pinned), 5 * (size_t)n * sizeof(float)));
-        buf.pcap = 5 * (size_t)n;
-    }
-    float* da = buf.dev;
-    float* dc = da + n;
-    float* dr = dc + n;
-    float* db = dr + n;
-    float* dx = db + n;
-    float* ws = dx + n;
-    float* hp = buf.pinned;  // same layout as the device arrays
-    const size_t bytes = sizeof(float) * (size_t)n;
-    // a call that failed after its upload may have left that copy queued
-    TPRH_HIP(hipStreamSynchronize(nullptr));
+    HostCtxLease lease(device);
+    if (!lease.ctx) return fail(TPRH_ERROR_OUT_OF_MEMORY, "host path context");
+    HostCtx& cx = *lease.ctx;
+    const bool zero_copy = !pl.split && n <= kZeroCopyMaxN;
+    const size_t nn = (size_t)n;
+    const size_t ws_floats = (pl.ws + sizeof(float) - 1) / sizeof(float);
+    tprh_status st = host_ctx_ready(cx, 5 * nn, zero_copy ? 0 : 5 * nn + ws_floats);
+    if (st != TPRH_SUCCESS) return st;
+    float* hp = cx.pinned;  // a | c | rhs | diag | x
+    const size_t bytes = sizeof(float) * nn;
     std::memcpy(hp, a, bytes);
-    std::memcpy(hp + n, c, bytes);
-    std::memcpy(hp + 2 * (size_t)n, rhs, bytes);
-    if (diag) std::memcpy(hp + 3 * (size_t)n, diag, bytes);
-    TPRH_HIP(hipMemcpyAsync(da, hp, (diag ? 4 : 3) * bytes, hipMemcpyHostToDevice, nullptr));
-    EventPair timing;
-    hipEvent_t* ev = timing.ev;
-    if (g_timing) {
-        TPRH_HIP(hipEventCreate(&ev[0]));
-        TPRH_HIP(hipEventCreate(&ev[1]));
-        TPRH_HIP(hipEventRecord(ev[0], nullptr));
-    }
-    tprh_status st = TPRH_SUCCESS;
+    std::memcpy(hp + nn, c, bytes);
+    std::memcpy(hp + 2 * nn, rhs, bytes);
+    if (diag) std::memcpy(hp + 3 * nn, diag, bytes);
+    // kernel operands: the mapped pinned buffer itself, or device copies
+    float* base = zero_copy ? cx.pinned_d : cx.dev;
+    float* da = base;
+    float* dc = da + nn;
+    float* dr = dc + nn;
+    float* db = dr + nn;
+    float* dx = db + nn;
+    float* ws = cx.dev ? cx.dev + 5 * nn : nullptr;
+    cx.dirty = true;
+    if (!zero_copy) TPRH_HIP(hipMemcpyAsync(da, hp, (diag ? 4 : 3) * bytes, hipMemcpyHostToDevice, cx.stream));
+    if (g_timing) TPRH_HIP(hipEventRecord(cx.ev[0], cx.stream));
     hipError_t le = hipSuccess;
     if (method == TPRH_PTPR || method == TPRH_TPR) {
         desc.a = da;
@@ -327,23 +372,21 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
         desc.x = dx;
         desc.workspace = ws;
         desc.workspace_bytes = pl.ws;
-        st = solve_desc(desc, nullptr);
+        st = solve_desc(desc, cx.stream);
     } else {
-        le = tprh::launch_single(method, da, diag ? db : nullptr, dc, dr, dx, n, 1, nullptr);
-    }
-    g_last_kernel_ms = -1.0f;
-    if (g_timing) {
-        hipError_t e1 = hipEventRecord(ev[1], nullptr);
-        if (st == TPRH_SUCCESS && le == hipSuccess && e1 == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess) {
-            float ms = -1.0f;
-            if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) g_last_kernel_ms = ms;
-        }
+        le = tprh::launch_single(method, da, diag ? db : nullptr, dc, dr, dx, n, 1, cx.stream);
     }
     if (st != TPRH_SUCCESS) return st;
     TPRH_HIP(le);
-    TPRH_HIP(hipMemcpyAsync(hp + 4 * (size_t)n, dx, bytes, hipMemcpyDeviceToHost, nullptr));
-    TPRH_HIP(hipStreamSynchronize(nullptr));
-    std::memcpy(x, hp + 4 * (size_t)n, bytes);
+    if (g_timing) TPRH_HIP(hipEventRecord(cx.ev[1], cx.stream));
+    if (!zero_copy) TPRH_HIP(hipMemcpyAsync(hp + 4 * nn, dx, bytes, hipMemcpyDeviceToHost, cx.stream));
+    TPRH_HIP(hipStreamSynchronize(cx.stream));
+    cx.dirty = false;
+    if (g_timing) {
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, cx.ev[0], cx.ev[1]) == hipSuccess) g_last_kernel_ms = ms;
+    }
+    std::memcpy(x, hp + 4 * nn, bytes);
     return TPRH_SUCCESS;
 }
 
@@ -363,6 +406,31 @@ namespace tprh {
 
 tprh_status set_error(tprh_status st, const std::string& msg) { return fail(st, msg); }
 tprh_status set_hip_error(hipError_t e, const char* what) { return fail_hip(e, what); }
+
+// tprh_solve_ex's plan for a caller whose workspace holds at most `cap`
+// bytes, without touching the device: *chunk is the round size to solve
+// with (the caller's, or with chunk 0 the default cut down until a round's
+// workspace fits) and *ws_bytes what that round needs (0: no workspace).
+// An explicit chunk whose round does not fit is an error.
+tprh_status plan_within(const tprh_solve_desc& d, size_t cap, int* chunk, size_t* ws_bytes) {
+    Plan pl;
+    tprh_status st = plan_desc(d, &pl);
+    if (st != TPRH_SUCCESS) return st;
+    if (pl.split && pl.ws > cap) {
+        const size_t one = tprh::split_workspace_bytes(pl.m, 1);
+        if (d.chunk > 0 || one > cap)
+            return fail(TPRH_ERROR_NOT_SUPPORTED,
+                        "a round of " + std::to_string(d.chunk > 0 ? pl.chunk : 1) + " systems needs " +
+                            std::to_string(d.chunk > 0 ? pl.ws : one) + " bytes of workspace; the fixed buffer holds " +
+                            std::to_string(cap));
+        const size_t per = one - tprh::split_workspace_bytes(pl.m, 0);
+        pl.chunk = (int)((cap - tprh::split_workspace_bytes(pl.m, 0)) / per);
+        pl.ws = tprh::split_workspace_bytes(pl.m, pl.chunk);
+    }
+    *chunk = pl.chunk;
+    *ws_bytes = pl.ws;
+    return TPRH_SUCCESS;
+}
 
 // Slice size and shape of the size class n under a 32-row (s, E, L) schedule
 // (NULL or a zero row: s = min(n, 256), default shape).  *split: the class

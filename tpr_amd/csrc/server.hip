@@ -904,8 +904,12 @@ tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_
         if (it->second.state == kDone) return TPRH_SUCCESS;
         if (it->second.state == kFailed) return tprh::set_error(it->second.status, sv->worker_error);
         if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+            // a request that completed or failed right at the deadline
+            // reports its own outcome, not the timeout
             auto jt = sv->reqs.find(request);
             if (jt != sv->reqs.end() && jt->second.state == kDone) return TPRH_SUCCESS;
+            if (jt != sv->reqs.end() && jt->second.state == kFailed)
+                return tprh::set_error(jt->second.status, sv->worker_error);
             return tprh::set_error(TPRH_ERROR_TIMEOUT, "timed out waiting for the request");
         }
     }
