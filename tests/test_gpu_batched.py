@@ -129,22 +129,25 @@ def test_alternative_shapes_bit_identical(s, E, L, method):
 def test_every_tuner_candidate_both_modes(method, s, m):
     """Every slice shape the batched tuner times (E * L = s, one workgroup
     per system) up to 16 slices per system: exact answers bit-identical to
-    the oracle; fast answers within 1e-5 of it and bit-identical to the
-    default shape's (the fast stage 1 has one operation order for every
-    shape), on four input kinds."""
-    from tpr_amd.autotune import batched_candidates, default_batched_shape
+    the oracle; fast answers within 1e-5 of it, and bit-identical to the
+    default shape's wherever the shape runs the same fast stage 1 (one
+    operation order per form: the two-level form of PTPR on 16-lane slices,
+    the register PCR everywhere else), on four input kinds."""
+    from tpr_amd.autotune import batched_candidates, default_batched_shape, fast_form
     n, nsys = s * m, 3
+    default = default_batched_shape(s)
     for kind in (O.GEN_DOMINANT, O.GEN_WEAK, O.GEN_FIXED, O.GEN_RANDOM_RHS):
         a, b, c, d, _ = O.gen_batch(5 + kind, 0, nsys, n, kind)
         ref = O.solve_batch(method, a, b, c, d, n, s)
-        x_def = gpu_solve(a, b, c, d, n, s, method, "fast", shape=default_batched_shape(s))
+        x_def = gpu_solve(a, b, c, d, n, s, method, "fast", shape=default)
         for shape in batched_candidates(n, s):
             say("candidate", method, n, s, shape, kind)
             assert bits_equal(gpu_solve(a, b, c, d, n, s, method, "exact", shape=shape), ref)
             x = gpu_solve(a, b, c, d, n, s, method, "fast", shape=shape)
             if kind != O.GEN_WEAK:  # weakly dominant systems (condition up to ~1e4) exceed 1e-5 in fp32
                 assert per_system_rel(x, ref, n).max() <= FAST_TOL
-            assert bits_equal(x, x_def)
+            if fast_form(method, *shape) == fast_form(method, *default):
+                assert bits_equal(x, x_def)
 
 
 def _tuned_rows_off_default():

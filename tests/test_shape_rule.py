@@ -47,13 +47,20 @@ def test_choice_keeps_the_answers_and_is_fastest(key, row):
         assert all(row["same_bits"].values())  # exact answers never depend on the shape
 
 
-def test_every_shape_gives_the_same_bits():
-    """Every candidate gave the default shape's answers on the tuning inputs,
-    in both arithmetic modes (the fast stage 1 has one operation order for
-    every shape), so no row's choice is limited by its answers; config 1
-    (fast PTPR, 4096 x 1024, s = 256) keeps 16 x 16, its fastest shape."""
+def test_same_bits_follow_the_arithmetic_form():
+    """On the tuning inputs, exact answers were the default shape's for every
+    candidate, and fast answers exactly for the candidates that run the
+    default's form of the fast stage 1 (tpr_amd.autotune.fast_form); config
+    1 (fast PTPR, 4096 x 1024, s = 256) keeps 16 x 16."""
     for key, row in ROWS:
-        assert all(row["same_bits"].values()), key
+        method, arith, n, s, nsys = A.parse_batched_key(key)
+        if arith == "exact":
+            assert all(row["same_bits"].values()), key
+            continue
+        dform = A.fast_form(method, *A.default_batched_shape(s))
+        for shape, same in row["same_bits"].items():
+            E, L = (int(v) for v in shape.split("x"))
+            assert same == (A.fast_form(method, E, L) == dform), (key, shape)
     assert slice_shape(1024, 256, 4096, "ptpr", "fast") == (16, 16)
 
 

@@ -114,6 +114,14 @@ def default_batched_shape(s):
     return {512: (16, 32), 1024: (16, 64), 2048: (32, 64)}[s]
 
 
+def fast_form(method, E, L):
+    """Which fast-arithmetic stage 1 a slice shape runs: PTPR on slices of
+    one 16-lane DPP row with E >= 2 takes the two-level form (a Thomas sweep
+    in each lane, PCR across the lanes' last rows), every other shape the
+    register PCR; shapes of one form give the same bits."""
+    return "two-level" if method == "ptpr" and L == 16 and E >= 2 else "pcr"
+
+
 def batched_candidates(n, s):
     """(E, L) shapes the batched entry point can run for (n, s) in one workgroup."""
     m = n // s

@@ -48,12 +48,13 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
         load_x_with_halo<E>(x, x0, off, n, xo, xl, xr);
         residual_fp64<E, HAS_B>(xo, xl, xr, raw);
     }
-    float A[E], C[E], R[E], X[E];
-    normalise<E, MODE>(raw, HAS_B, A, C, R);
+    float X[E];
     constexpr bool WAVE = BLOCK == kSmallBlock;
     if constexpr (METHOD == 0) {
-        ptpr_solve_regs<E, L, MODE, WAVE>(A, C, R, m, t, my, X);
+        ptpr_solve_raw<E, L, MODE, WAVE>(raw, HAS_B, m, t, my, X);
     } else {
+        float A[E], C[E], R[E];
+        normalise<E, MODE>(raw, HAS_B, A, C, R);
         tpr_solve_regs<E, L, MODE, WAVE>(A, C, R, m, t, my, X);
     }
     if constexpr (REFINE) {

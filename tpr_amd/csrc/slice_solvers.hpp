@@ -600,22 +600,138 @@ __device__ __forceinline__ void tpr_solve_regs(float (&A)[E], float (&C)[E], flo
     tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xr[j], l, X);
 }
 
-template <int E, int L, int MODE>
-__device__ __forceinline__ void ptpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
-                                                 size_t off, int m, int t, bool sys_valid, float* lds) {
-    float A[E], C[E], R[E], X[E];
-    normalise<E, MODE>(raw, has_b, A, C, R);
-    ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, lds, X);
-    if (sys_valid) store_e<E>(x + off, X);
+// ---------------------------------------------------------------------------
+// PTPR, Fast policy, slices of one DPP row (L == 16) with E >= 2 equations per
+// lane.  Stage 1's result -- every row of the slice as
+//   x_i = R_i - A_i x[st-1] - C_i x[ed+1]
+// -- is the same linear map whatever order eliminates the slice's interior;
+// here it is reached in two levels instead of log2(S) PCR levels over every
+// equation:
+//   (1) in each lane, the first E-1 rows are eliminated by a Thomas sweep
+//       (forward elimination, back-substitution of three right-hand sides):
+//         x_k = y_k + p_k e_{l-1} + q_k e_l,   k = 0 .. E-2,
+//       where e_l is the lane's last unknown and e_{l-1} the previous
+//       lane's (x[st-1] for the slice's first lane); the division by the
+//       diagonal is part of the sweep;
+//   (2) each lane's last row, with its neighbours' x_k substituted, is one
+//       row of a tridiagonal system in the e's (x[st-1] and x[ed+1] at its
+//       ends), the slice's 16 lanes one row each; PTPR's own stage 1 on
+//       that system -- the in-slice PCR levels at lane distances 1, 2, 4, 8,
+//       through the fused DPP forms -- leaves e_l = R_l - A_l x[st-1] -
+//       C_l x[ed+1].
+// The slice's last row is lane 15's e, its first row follows from lane 0's
+// x_0; the reduced system (src/ptpr.cpp:193-258) is solved as for every
+// other shape.  Stage 3 (src/ptpr.cpp:263-287) computes key = x[ed+1] from
+// the slice's last row as the reference does, then every lane its e_l and
+// its rows x_k = y_k + p_k e_{l-1} + q_k e_l.
+// Work per lane: about 9 VALU and one reciprocal per equation in (1), and
+// four cross-lane levels of one row in (2), against log2(S) levels of 9 VALU
+// and a reciprocal per equation in the PCR form.  Mathematically the same
+// stage 1; rounding differs from the reference's order, which the exact
+// policy keeps.
+// The lane's state between the two halves of the two-level solve.
+template <int E>
+struct Fast16State {
+    float y[E - 1], pc[E - 1], q[E - 1];  // x_k = y_k + pc_k e_{l-1} + q_k e_l
+    float eA, eC, eR;                     // e_l = eR - eA x[st-1] - eC x[ed+1]
+    float sA, sC, sR;                     // the slice's first row (valid in lane 0)
+};
+
+// First half: stages (1) and (2) above, from the raw loads.
+template <int E>
+__device__ __forceinline__ void fast16_stage1(const RawPart<E>& p, bool has_b, int l, Fast16State<E>& st) {
+    static_assert(E >= 2, "the lane needs at least one interior row");
+    // (1) forward elimination over rows 0 .. E-2: c' in cp, y' in y, p' in pc
+    float cp[E - 1];
+    float(&y)[E - 1] = st.y;
+    float(&pc)[E - 1] = st.pc;
+    float(&q)[E - 1] = st.q;
+    {
+        const float inv = fast_rcp(has_b ? p.b[0] : 1.0f);
+        cp[0] = p.c[0] * inv;
+        y[0] = p.d[0] * inv;
+        pc[0] = -p.a[0] * inv;
+    }
+#pragma unroll
+    for (int k = 1; k < E - 1; k++) {
+        const float den = __builtin_fmaf(-p.a[k], cp[k - 1], has_b ? p.b[k] : 1.0f);
+        const float inv = fast_rcp(den);
+        cp[k] = p.c[k] * inv;
+        y[k] = __builtin_fmaf(-p.a[k], y[k - 1], p.d[k]) * inv;
+        pc[k] = (-p.a[k] * pc[k - 1]) * inv;
+    }
+    // back-substitution: y_k, p_k (coefficient of e_{l-1}), q_k (of e_l)
+    q[E - 2] = -cp[E - 2];
+#pragma unroll
+    for (int k = E - 3; k >= 0; k--) {
+        y[k] = __builtin_fmaf(-cp[k], y[k + 1], y[k]);
+        pc[k] = __builtin_fmaf(-cp[k], pc[k + 1], pc[k]);
+        q[k] = -cp[k] * q[k + 1];
+    }
+    // (2) the lane's last row: a x_{E-2} + b e_l + c x_E = d with x_E the next
+    // lane's x_0 = y0' + p0' e_l + q0' e_{l+1}; past the slice's last lane
+    // x_E is x[ed+1] itself (y0' = p0' = 0, q0' = 1)
+    const float yn = shift_down<16, 1>(y[0], 0.0f, l);
+    const float pn = shift_down<16, 1>(pc[0], 0.0f, l);
+    const float qn = shift_down<16, 1>(q[0], 1.0f, l);
+    const float aL = p.a[E - 1], cL = p.c[E - 1];
+    const float beta = __builtin_fmaf(cL, pn, __builtin_fmaf(aL, q[E - 2], has_b ? p.b[E - 1] : 1.0f));
+    const float delta = __builtin_fmaf(-cL, yn, __builtin_fmaf(-aL, y[E - 2], p.d[E - 1]));
+    const float ib = fast_rcp(beta);
+    // negated off-diagonals, the form the fused DPP levels take
+    float Ap[1] = {-((aL * pc[E - 2]) * ib)};
+    float Cp[1] = {-((cL * qn) * ib)};
+    float Rr[1] = {delta * ib};
+    ptpr_stage1_fast16<1, 1>(Ap, Cp, Rr);
+    st.eA = -Ap[0];
+    st.eC = -Cp[0];
+    st.eR = Rr[0];
+    // slice's first row from lane 0's x_0 = y_0 + p_0 x[st-1] + q_0 e_0
+    st.sA = __builtin_fmaf(q[0], st.eA, -pc[0]);
+    st.sC = q[0] * st.eC;
+    st.sR = __builtin_fmaf(q[0], st.eR, y[0]);
 }
 
-template <int E, int L, int MODE>
-__device__ __forceinline__ void tpr_solve_lanes(const RawPart<E>& raw, bool has_b, float* __restrict__ x,
-                                                size_t off, int m, int t, bool sys_valid, float* lds) {
-    float A[E], C[E], R[E], X[E];
-    normalise<E, MODE>(raw, has_b, A, C, R);
-    tpr_solve_regs<E, L, MODE>(A, C, R, m, t, lds, X);
-    if (sys_valid) store_e<E>(x + off, X);
+// Second half: stage 3 from x[st-1] = xprev, x[ed] = xe and the slice's
+// last row se (as the reference's stage 3 takes them).
+template <int E>
+__device__ __forceinline__ void fast16_stage3(const Fast16State<E>& st, float xprev, float xe, SliceEnd se, int l,
+                                              float (&X)[E]) {
+    float key = fast_rcp(se.c) * (__builtin_fmaf(-se.a, xprev, se.r) - xe);
+    if (se.c == 0.0f) key = 0.0f;
+    float e = __builtin_fmaf(-st.eC, key, __builtin_fmaf(-st.eA, xprev, st.eR));
+    if (l == 15) e = xe;
+    const float eprev = shift_up<16, 1>(e, xprev, l);
+#pragma unroll
+    for (int k = 0; k < E - 1; k++) X[k] = __builtin_fmaf(st.q[k], e, __builtin_fmaf(st.pc[k], eprev, st.y[k]));
+    X[E - 1] = e;
+}
+
+template <int E, bool WAVE>
+__device__ __forceinline__ void ptpr_solve_fast16_partition(const RawPart<E>& p, bool has_b, int m, int t,
+                                                            float* lds, float (&X)[E]) {
+    const int l = t & 15;
+    const int j = t >> 4;
+    Fast16State<E> st;
+    fast16_stage1<E>(p, has_b, l, st);
+    const float* xr = reduced_solve<0, WAVE>(lds, m, j, l, 16, t, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR);
+    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+    fast16_stage3<E>(st, xprev, xr[j], slice_end(lds, m, j), l, X);
+}
+
+// Whole PTPR solve from the raw loads (the division by the diagonal
+// included): the two-level form above where it applies, else the
+// normalisation followed by the register PCR of ptpr_solve_regs.
+template <int E, int L, int MODE, bool WAVE = false>
+__device__ __forceinline__ void ptpr_solve_raw(const RawPart<E>& raw, bool has_b, int m, int t, float* lds,
+                                               float (&X)[E]) {
+    if constexpr (MODE == kFast && L == 16 && E >= 2) {
+        ptpr_solve_fast16_partition<E, WAVE>(raw, has_b, m, t, lds, X);
+    } else {
+        float A[E], C[E], R[E];
+        normalise<E, MODE>(raw, has_b, A, C, R);
+        ptpr_solve_regs<E, L, MODE, WAVE>(A, C, R, m, t, lds, X);
+    }
 }
 
 }  // namespace tprh

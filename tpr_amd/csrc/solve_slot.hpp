@@ -26,12 +26,13 @@ __device__ __forceinline__ void solve_slot(int m, int lg_lanes, bool valid, cons
     const size_t off = (size_t)t * E;
     RawPart<E> raw;
     load_raw<E>(pa, HAS_B ? pb : nullptr, pc, pd, off, raw);
-    float A[E], C[E], R[E], X[E];
-    normalise<E, MODE>(raw, HAS_B, A, C, R);
+    float X[E];
     float* my = lds + slot * lds_words_per_system(m);
     if constexpr (METHOD == 0) {
-        ptpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
+        ptpr_solve_raw<E, L, MODE>(raw, HAS_B, m, t, my, X);
     } else {
+        float A[E], C[E], R[E];
+        normalise<E, MODE>(raw, HAS_B, A, C, R);
         tpr_solve_regs<E, L, MODE>(A, C, R, m, t, my, X);
     }
     if (valid) store_e<E>(px + off, X);
