@@ -246,6 +246,18 @@ typedef struct tprh_server_config {
      * max_wait_us (0, 0: launch whenever a slot is free). */
     int min_batch_systems;
     double max_wait_us;
+    /* 1: adaptive window instead of the fixed one.  While batches run, the
+     * next batch waits until the earliest of them is expected to finish
+     * within the time the server takes to build and launch a batch (device
+     * time per equation and launch time are measured as batches complete),
+     * or until it holds max_batch_systems right-hand sides, or (when
+     * max_wait_us > 0) until its oldest request has waited max_wait_us;
+     * min_batch_systems is not used.  The device stays busy and each batch
+     * takes what arrived while the others ran.  It helps where the device
+     * is the bottleneck; a server bound by host work per request (small
+     * batches that finish before the next is formed) sees no change.  0:
+     * the fixed window above. */
+    int adaptive_window;
 } tprh_server_config;
 
 tprh_status tprh_server_create(tprh_server **out, const tprh_server_config *cfg);

@@ -468,6 +468,7 @@ int main(int argc, char** argv) {
     cfg.max_batch_systems = 24;
     cfg.min_batch_systems = seed % 2 ? 8 : 0;
     cfg.max_wait_us = seed % 2 ? 300.0 : 0.0;
+    cfg.adaptive_window = (seed / 2) % 2;  // seeds 2, 3: the adaptive window
     tprh_server* sv = nullptr;
     check_ok(tprh_server_create(&sv, &cfg), "create");
     if (!sv) return 2;

@@ -83,7 +83,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("arith", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("max_inflight", ctypes.c_int), ("max_batch_systems", ctypes.c_int),
                 ("schedule", ctypes.c_void_p), ("min_batch_systems", ctypes.c_int),
-                ("max_wait_us", ctypes.c_double)]
+                ("max_wait_us", ctypes.c_double), ("adaptive_window", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):

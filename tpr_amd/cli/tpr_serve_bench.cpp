@@ -3,7 +3,9 @@
 //   tpr_serve_bench [--rate R] [--seconds T] [--requests K] [--outstanding Q]
 //                   [--clients C] [--min-log2 A] [--max-log2 B] [--max-rhs H]
 //                   [--budget-gb G] [--inflight F] [--min-batch M] [--max-wait-us W]
+//                   [--adaptive 0|1]
 //                   [--method ptpr|tpr] [--arith fast|exact] [--seed S]
+//                   [--dump FILE --dump-every D]
 //
 // Open loop (--rate R > 0): requests arrive as a Poisson stream of mean R per
 // second for T seconds.  Closed loop (--rate 0): K requests, at most Q
@@ -19,6 +21,11 @@
 // loop each keeps at most Q / C outstanding) and one collector thread takes
 // the completions (sleeping in tprh_server_wait_any), records latencies and
 // releases; the server's calls are thread-safe.  Prints one JSON line.
+// --dump: the answers of every D-th request of the schedule (k % D == 0) are
+// copied out before their release and appended to FILE as records of four
+// int64 (k, n, nrhs, j0) followed by nrhs * n floats, so that a test can
+// check them against an independent solve; the copy runs on the collector
+// thread, so a dumping run's latencies are not benchmark figures.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -46,9 +53,12 @@ struct Args {
     int inflight = 4;
     int min_batch = 0;
     double max_wait_us = 0.0;
+    int adaptive = 0;
     tprh_method method = TPRH_PTPR;
     tprh_arith arith = TPRH_ARITH_FAST;
     unsigned long long seed = 4044;
+    std::string dump;
+    long long dump_every = 0;
 };
 
 [[noreturn]] void usage(const char* msg) {
@@ -74,7 +84,10 @@ Args parse(int argc, char** argv) {
         else if (k == "--inflight") a.inflight = std::atoi(v);
         else if (k == "--min-batch") a.min_batch = std::atoi(v);
         else if (k == "--max-wait-us") a.max_wait_us = std::atof(v);
+        else if (k == "--adaptive") a.adaptive = std::atoi(v);
         else if (k == "--seed") a.seed = std::strtoull(v, nullptr, 10);
+        else if (k == "--dump") a.dump = v;
+        else if (k == "--dump-every") a.dump_every = std::atoll(v);
         else if (k == "--method") {
             if (!std::strcmp(v, "ptpr")) a.method = TPRH_PTPR;
             else if (!std::strcmp(v, "tpr")) a.method = TPRH_TPR;
@@ -88,6 +101,7 @@ Args parse(int argc, char** argv) {
     if (a.min_log2 < 4 || a.max_log2 < a.min_log2 || a.max_rhs < 1) usage("bad size or right-hand-side range");
     if (a.clients < 1 || a.clients > 64) usage("clients must be 1..64");
     if (a.outstanding < a.clients) usage("outstanding must be at least the number of clients");
+    if (!a.dump.empty() && a.dump_every < 1) usage("--dump needs --dump-every >= 1");
     return a;
 }
 
@@ -119,6 +133,7 @@ int main(int argc, char** argv) {
     cfg.max_batch_systems = 16384;
     cfg.min_batch_systems = a.min_batch;
     cfg.max_wait_us = a.max_wait_us;
+    cfg.adaptive_window = a.adaptive;
     tprh_server* srv = nullptr;
     check(tprh_server_create(&srv, &cfg), "tprh_server_create");
 
@@ -170,11 +185,16 @@ int main(int argc, char** argv) {
     // (matrix + 1, 0 while unset) and read by the collector
     std::vector<std::atomic<long long>> owner_mat(nreq);
     std::vector<int> owner_client(nreq);
+    std::vector<long long> owner_k(nreq);  // which request of the schedule
     for (auto& o : owner_mat) o.store(0);
     std::vector<std::atomic<int>> outstanding(a.clients);
     for (auto& o : outstanding) o.store(0);
     const int per_client = a.outstanding / a.clients;
     std::vector<double> lat, qwait, dev;  // submit->done, submit->launch, launch->done
+    // open loop: scheduled arrival->done, so that a client that falls behind
+    // its schedule (and submits late) does not hide that delay
+    std::vector<double> lat_arr;
+    lat_arr.reserve(nreq);
     lat.reserve(nreq);
     qwait.reserve(nreq);
     dev.reserve(nreq);
@@ -202,9 +222,15 @@ int main(int argc, char** argv) {
                 std::exit(1);
             }
             owner_client[r - first_id] = c;
+            owner_k[r - first_id] = k;
             owner_mat[r - first_id].store(m + 1, std::memory_order_release);
         }
     };
+    FILE* dump = nullptr;
+    if (!a.dump.empty() && !(dump = std::fopen(a.dump.c_str(), "wb"))) {
+        std::fprintf(stderr, "cannot open %s\n", a.dump.c_str());
+        return 1;
+    }
     auto collector = [&]() {
         std::vector<long long> ids(4096);
         long long done = 0;
@@ -225,7 +251,20 @@ int main(int argc, char** argv) {
                     failed.store(true);
                     return;
                 }
+                const long long kk = owner_k[r - first_id];
+                if (dump && kk % a.dump_every == 0) {
+                    std::vector<float> xs((size_t)inf.nrhs * (size_t)inf.n);
+                    check(tprh_server_copy_result(srv, r, xs.data()), "copy_result");
+                    const long long hdr[4] = {kk, inf.n, inf.nrhs, 0};
+                    if (std::fwrite(hdr, sizeof(long long), 4, dump) != 4 ||
+                        std::fwrite(xs.data(), sizeof(float), xs.size(), dump) != xs.size()) {
+                        std::fprintf(stderr, "writing %s failed\n", a.dump.c_str());
+                        failed.store(true);
+                        return;
+                    }
+                }
                 lat.push_back(inf.t_done - inf.t_submit);
+                if (open) lat_arr.push_back(inf.t_done - (t0 + arrive[owner_k[r - first_id]]));
                 qwait.push_back(inf.t_launch - inf.t_submit);
                 dev.push_back(inf.t_done - inf.t_launch);
                 check(tprh_server_request_release(srv, r), "release");
@@ -240,6 +279,7 @@ int main(int argc, char** argv) {
     for (int c = 0; c < a.clients; c++) cl.emplace_back(client, c);
     for (auto& th : cl) th.join();
     coll.join();
+    if (dump && std::fclose(dump) != 0) failed.store(true);
     if (failed.load()) return 1;
     const double late_max = 1e-9 * (double)late_ns.load();
     const double wall = tprh_server_now(srv) - t0;
@@ -255,15 +295,20 @@ int main(int argc, char** argv) {
         "{\"mode\": \"%s\", \"rate\": %.1f, \"requests\": %lld, \"systems\": %lld, \"equations\": %lld, "
         "\"wall_s\": %.4f, \"requests_per_s\": %.1f, \"systems_per_s\": %.1f, \"GBps\": %.1f, "
         "\"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"max\": %.4f}, "
+        "\"latency_from_arrival_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"max\": %.4f}, "
         "\"queued_ms\": {\"p50\": %.4f, \"p99\": %.4f}, \"on_device_ms\": {\"p50\": %.4f, \"p99\": %.4f}, "
         "\"max_arrival_lag_ms\": %.4f, "
         "\"batches\": %lld, \"max_batch_systems_seen\": %lld, \"max_inflight_seen\": %d, \"peak_bytes\": %zu, "
-        "\"outstanding\": %d, \"clients\": %d, \"min_batch\": %d, \"max_wait_us\": %.1f}\n",
+        "\"outstanding\": %d, \"clients\": %d, \"min_batch\": %d, \"max_wait_us\": %.1f, \"adaptive\": %d}\n",
         open ? "poisson" : "closed", a.rate, nreq, systems, equations, wall, (double)nreq / wall,
         (double)systems / wall, bytes / wall / 1e9, 1e3 * pct(lat, 50), 1e3 * pct(lat, 99),
-        1e3 * (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end())), 1e3 * pct(qwait, 50),
+        1e3 * (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end())),
+        1e3 * pct(open ? lat_arr : lat, 50), 1e3 * pct(open ? lat_arr : lat, 99),
+        1e3 * (open ? (lat_arr.empty() ? 0.0 : *std::max_element(lat_arr.begin(), lat_arr.end()))
+                    : (lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end()))),
+        1e3 * pct(qwait, 50),
         1e3 * pct(qwait, 99), 1e3 * pct(dev, 50), 1e3 * pct(dev, 99), 1e3 * late_max, st.batches,
         st.max_batch_systems_seen, st.max_inflight_seen, st.peak_bytes, open ? 0 : a.outstanding, a.clients,
-        a.min_batch, a.max_wait_us);
+        a.min_batch, a.max_wait_us, a.adaptive);
     return 0;
 }

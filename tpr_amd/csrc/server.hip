@@ -153,6 +153,8 @@ struct Slot {
     bool busy = false;
     std::vector<long long> reqs;
     long long systems = 0, equations = 0;
+    double t_launched = 0;  // when launch_batch returned (server clock)
+    double predicted = 0;   // its expected time on the device, s (adaptive window)
 };
 
 }  // namespace
@@ -188,6 +190,11 @@ struct tprh_server {
     std::string worker_error;
 
     tprh_server_stats st{};
+    // adaptive batching window (worker thread only): device time per
+    // equation of a batch, and host time to build and launch one, as
+    // exponentially weighted averages over the batches seen so far
+    double s_per_eq = 0.0;
+    double launch_s = 50e-6;
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -457,6 +464,30 @@ hipError_t launch_batch(tprh_server* sv, Slot& sl, const std::vector<Staged>& it
     return hipEventRecord(sl.done, sl.stream);
 }
 
+// While batches run, whether the next batch should wait for more requests
+// (called by the worker under the lock, queue not empty).
+// Fixed window: until min_batch_systems right-hand sides are queued or the
+// oldest queued request has waited max_wait_us.
+// Adaptive window: until the earliest in-flight batch is expected to finish
+// within the time it takes to build and launch a batch -- so the device is
+// kept busy and each batch takes everything that arrived while the previous
+// ones ran -- or a batch is full, or (when max_wait_us > 0) the oldest request
+// has waited max_wait_us.  Expected finish times come from the measured
+// device time per equation of the batches seen so far; before the first one
+// has completed nothing waits.
+bool batch_should_wait(tprh_server* sv) {
+    const double now = sv->now();
+    const double waited_us = 1e6 * (now - sv->reqs[sv->queue.front()].t_submit);
+    if (!sv->cfg.adaptive_window)
+        return sv->queued_systems < sv->cfg.min_batch_systems && waited_us < sv->cfg.max_wait_us;
+    if (sv->s_per_eq <= 0.0 || sv->queued_systems >= sv->cfg.max_batch_systems) return false;
+    if (sv->cfg.max_wait_us > 0.0 && waited_us >= sv->cfg.max_wait_us) return false;
+    double first_free = 1e300;
+    for (const Slot& sl : sv->slots)
+        if (sl.busy) first_free = std::min(first_free, sl.t_launched + sl.predicted);
+    return now + sv->launch_s < first_free;
+}
+
 // The worker polls its in-flight batches without the lock (a slot's state
 // changes only on this thread) and takes the lock only when a batch has
 // finished, when a slot is free and a submit or a release has happened since
@@ -491,14 +522,21 @@ void worker_main(tprh_server* sv) {
         std::unique_lock<std::mutex> lk(sv->mu);
         seen = wake;
         recheck = false;
-        for (const auto& f : fired) retire_locked(sv, sv->slots[f.first], f.second);
+        for (const auto& f : fired) {
+            Slot& sl = sv->slots[f.first];
+            if (f.second == hipSuccess && sl.equations > 0) {
+                // the batch's time from launch to the completion seen here
+                const double rate = (sv->now() - sl.t_launched) / (double)sl.equations;
+                sv->s_per_eq = sv->s_per_eq > 0.0 ? 0.8 * sv->s_per_eq + 0.2 * rate : rate;
+            }
+            retire_locked(sv, sl, f.second);
+        }
         if (!fired.empty()) sv->cv_done.notify_all();
         // 2. admission and launch
         bool launched = false;
         for (Slot& sl : sv->slots) {
             if (sl.busy || sv->queue.empty()) continue;
-            if (inflight > 0 && sv->queued_systems < sv->cfg.min_batch_systems &&
-                1e6 * (sv->now() - sv->reqs[sv->queue.front()].t_submit) < sv->cfg.max_wait_us) {
+            if (inflight > 0 && batch_should_wait(sv)) {
                 recheck = true;
                 break;  // batches are running: let this one grow
             }
@@ -554,7 +592,11 @@ void worker_main(tprh_server* sv) {
                 items.push_back(it);
             }
             lk.unlock();  // the HIP calls of the launch run without the lock
+            const double t_begin = sv->now();
             hipError_t e = launch_batch(sv, sl, items);
+            sl.t_launched = sv->now();
+            sv->launch_s = 0.8 * sv->launch_s + 0.2 * (sl.t_launched - t_begin);
+            sl.predicted = sv->s_per_eq * (double)equations;
             lk.lock();
             if (e != hipSuccess) {
                 (void)hipStreamSynchronize(sl.stream);
@@ -632,6 +674,8 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
     if (cfg->hbm_budget < kAlign) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "hbm_budget too small");
     if (cfg->min_batch_systems < 0 || !(cfg->max_wait_us >= 0.0))
         return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "batching window must be non-negative");
+    if (cfg->adaptive_window != 0 && cfg->adaptive_window != 1)
+        return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "adaptive_window must be 0 or 1");
     if (cfg->schedule) {
         // every row must name a shape the serve kernel has (or be zero)
         for (int lg = 4; lg < 32; lg++) {

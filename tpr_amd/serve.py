@@ -39,12 +39,13 @@ class Server:
     """tprh_server: a request loop over one arena of ``budget_bytes``."""
 
     def __init__(self, budget_bytes, method="ptpr", arith="fast", max_inflight=4, max_batch_systems=16384,
-                 schedule=None, min_batch_systems=0, max_wait_us=0.0):
+                 schedule=None, min_batch_systems=0, max_wait_us=0.0, adaptive_window=False):
         cfg = _lib.ServerConfig()
         cfg.method, cfg.arith = METHODS[method], ARITH[arith]
         cfg.hbm_budget = int(budget_bytes)
         cfg.max_inflight, cfg.max_batch_systems = int(max_inflight), int(max_batch_systems)
         cfg.min_batch_systems, cfg.max_wait_us = int(min_batch_systems), float(max_wait_us)
+        cfg.adaptive_window = 1 if adaptive_window else 0
         self._tab = schedule_array(schedule) if schedule else None
         cfg.schedule = self._tab.ctypes.data if self._tab is not None else None
         h = ctypes.c_void_p()
