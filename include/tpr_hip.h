@@ -177,6 +177,14 @@ typedef struct tprh_solve_desc {
     size_t workspace_bytes;
     int coef_shared;        /* 1: every system uses the one matrix at a, b, c (n values each):
                                nsys right-hand sides of one matrix, as in ADI sweeps */
+    int split_form;         /* systems above one workgroup: 0 = chosen by the library (two
+                               passes); 1 = two passes (stage 1 and the reduced systems, then
+                               stage 3, which reads the inputs again: 36 B/eq); 2 = one pass
+                               (each slice stays in registers from load to store, 20 B/eq, where
+                               every workgroup of a system can be resident at once, else two
+                               passes; slower than two passes when a round has more workgroups
+                               than the device holds).  Exact answers are the same bits in
+                               either form. */
 } tprh_solve_desc;
 
 /* Workspace bytes tprh_solve_ex needs for desc (0 when none); (size_t)-1 if

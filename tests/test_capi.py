@@ -108,9 +108,11 @@ def test_workspace_sizing_without_device():
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == 0          # one workgroup per system
     d.n, d.nsys = 65536, 256
     ws = L.tprh_workspace_bytes(ctypes.byref(d))
-    assert ws == 4 * 256 * 9 * 256 + 256                          # rows, boundary x and x_ed per slice
+    # rows, boundary x and x_ed per slice, and a 128-byte line of
+    # synchronisation words per system and one more
+    assert ws == 4 * 256 * 9 * 256 + 128 * (256 + 1) + 256
     d.chunk = 3
-    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 256
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 128 * (3 + 1) + 256
     d.s = 3
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
     d.s, d.refine, d.coef = 256, 1, _lib.STORE_F16                # refinement needs fp32 coefficients
@@ -118,7 +120,7 @@ def test_workspace_sizing_without_device():
     d.refine, d.coef, d.coef_shared = 0, 0, 2                      # coef_shared is a flag
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
     d.coef_shared = 1                                             # one matrix: same workspace per system
-    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 256
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 128 * (3 + 1) + 256
     # a large system without a workspace is refused before any launch
     d.refine, d.coef, d.coef_shared = 0, 0, 0
     d.a = d.b = d.c = d.d = d.x = 16

@@ -264,3 +264,35 @@ def test_solve_ex_side_stream_keeps_its_workspace():
         del junk
     torch.cuda.synchronize()
     assert bits_equal(x.cpu().numpy(), x_ref.cpu().numpy())
+
+
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nsys,chunk", [SPLIT_CASES[0], SPLIT_CASES[2], SPLIT_CASES[3], SPLIT_CASES[6],
+                                            (65536, 256, 70, 0)])
+def test_split_forms_bit_identical(form, method, n, s, nsys, chunk):
+    """Both forms of the multi-workgroup path, forced: two passes (stage 1
+    with last-arriver reduced solves, then stage 3) and one pass (slices in
+    registers throughout; 70 systems of 65536 are more blocks than the chip
+    holds at once, so its blocks come in several waves): exact answers
+    bit-identical to the oracle, fast ones within tolerance, with and
+    without a refinement step."""
+    say("split form", form, method, n, s, nsys, chunk)
+    a, b, c, d, xt = O.gen_batch(39, 0, nsys, n, O.GEN_WEAK)
+    da, db, dc, dd = dev(a), dev(b), dev(c), dev(d)
+    x = torch.empty(nsys * n, device="cuda")
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, arith="exact", chunk=chunk, split_form=form)
+    torch.cuda.synchronize()
+    x0 = x.cpu().numpy()
+    assert bits_equal(x0, O.solve_batch(method, a, b, c, d, n, s))
+    B.solve_ex(da, db, dc, dd, x, n, s, method=method, arith="exact", chunk=chunk, split_form=form, refine=True)
+    torch.cuda.synchronize()
+    if nsys <= 8:
+        assert bits_equal(x.cpu().numpy(), oracle_refine(method, a, b, c, d, x0, n, s))
+    xf = torch.empty_like(x)
+    ga, gb, gc, gd, _ = O.gen_batch(40, 0, nsys, n)
+    B.solve_ex(dev(ga), dev(gb), dev(gc), dev(gd), xf, n, s, method=method, arith="fast", chunk=chunk,
+               split_form=form)
+    torch.cuda.synchronize()
+    ref = O.solve_batch(method, ga, gb, gc, gd, n, s)
+    assert per_system_rel(xf.cpu().numpy(), ref, n).max() <= FAST_TOL

@@ -64,7 +64,7 @@ def _dptr_any(t, name, numel, dtypes):
 
 
 def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
-              shared=False):
+              shared=False, split_form=0):
     """tprh_solve_desc for torch tensors; a, b, c float32 or float16 (same
     dtype).  shared: a, b, c hold one matrix (n values) for every right-hand
     side in d."""
@@ -88,6 +88,7 @@ def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sh
     desc.E, desc.L = shape if shape is not None else (0, 0)
     desc.chunk = int(chunk)
     desc.coef_shared = 1 if shared else 0
+    desc.split_form = int(split_form)
     return desc
 
 
@@ -99,14 +100,16 @@ def workspace_bytes(desc):
 
 
 def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
-             workspace=None, stream=None, shared=False):
+             workspace=None, stream=None, shared=False, split_form=0):
     """The general entry point (tprh_solve_ex): large systems (through a
     workspace, allocated here from torch if not given), fp16 coefficient
     storage (a, b, c as float16 tensors), one refinement step (refine=True:
     x += A^-1 (d - A x) with an FP64 residual), many right-hand sides of one
-    matrix (shared=True: a, b, c of n values, d and x of nsys * n)."""
+    matrix (shared=True: a, b, c of n values, d and x of nsys * n); split_form
+    0 / 1 / 2: the library's choice / two passes / one pass for systems above
+    one workgroup."""
     import torch
-    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared)
+    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared, split_form)
     need = workspace_bytes(desc)
     own = None
     if need:

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -148,6 +149,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     if (d.coef != TPRH_STORE_F32 && d.coef != TPRH_STORE_F16) return fail(TPRH_ERROR_INVALID_VALUE, "bad storage");
     if (d.refine != 0 && d.refine != 1) return fail(TPRH_ERROR_INVALID_VALUE, "refine must be 0 or 1");
     if (d.coef_shared != 0 && d.coef_shared != 1) return fail(TPRH_ERROR_INVALID_VALUE, "coef_shared must be 0 or 1");
+    if (d.split_form < 0 || d.split_form > 2) return fail(TPRH_ERROR_INVALID_VALUE, "split_form must be 0, 1 or 2");
     if (d.refine && d.coef != TPRH_STORE_F32)
         return fail(TPRH_ERROR_INVALID_VALUE, "refinement needs fp32 coefficients for the residual");
     const bool explicit_shape = d.E != 0 || d.L != 0;
@@ -196,6 +198,7 @@ tprh_status solve_desc(const tprh_solve_desc& d, hipStream_t stream) {
         p.f16 = d.coef == TPRH_STORE_F16;
         p.refine = d.refine != 0;
         p.shared_coef = d.coef_shared != 0;
+        p.form = d.split_form;
         tprh::split_carve(d.workspace, pl.m, pl.chunk, &p);
         e = tprh::launch_split(method, mode, p, pl.E, pl.L, stream);
     }
@@ -240,6 +243,7 @@ struct HostCtx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t done = nullptr;  // completion of a call's work (no timing)
     float* pinned = nullptr;    // host address
     float* pinned_d = nullptr;  // the same memory as the device sees it
     size_t pcap = 0;            // floats
@@ -282,6 +286,7 @@ tprh_status host_ctx_ready(HostCtx& cx, size_t pinned_floats, size_t dev_floats)
         TPRH_HIP(hipStreamCreateWithFlags(&cx.stream, hipStreamNonBlocking));
         TPRH_HIP(hipEventCreate(&cx.ev[0]));
         TPRH_HIP(hipEventCreate(&cx.ev[1]));
+        TPRH_HIP(hipEventCreateWithFlags(&cx.done, hipEventDisableTiming));
     }
     if (cx.dirty) {
         TPRH_HIP(hipStreamSynchronize(cx.stream));
@@ -303,6 +308,23 @@ tprh_status host_ctx_ready(HostCtx& cx, size_t pinned_floats, size_t dev_floats)
         TPRH_HIP(hipMalloc(&cx.dev, dev_floats * sizeof(float)));
         cx.cap = dev_floats;
     }
+    return TPRH_SUCCESS;
+}
+
+// Waits for the work a call queued on its stream.  A solve of the sizes the
+// host-pointer entry points take completes within tens of microseconds, and
+// a sleeping wait would add its wake-up latency to every call, so the
+// completion event is polled first (for at most 2 ms), then waited for.
+tprh_status host_ctx_wait(HostCtx& cx) {
+    TPRH_HIP(hipEventRecord(cx.done, cx.stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t q = hipEventQuery(cx.done);
+        if (q == hipSuccess) return TPRH_SUCCESS;
+        if (q != hipErrorNotReady) return fail_hip(q, "hipEventQuery");
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    }
+    TPRH_HIP(hipEventSynchronize(cx.done));
     return TPRH_SUCCESS;
 }
 
@@ -380,7 +402,8 @@ tprh_status host_path_solve(tprh_method method, tprh_arith arith, const float* a
     TPRH_HIP(le);
     if (g_timing) TPRH_HIP(hipEventRecord(cx.ev[1], cx.stream));
     if (!zero_copy) TPRH_HIP(hipMemcpyAsync(hp + 4 * nn, dx, bytes, hipMemcpyDeviceToHost, cx.stream));
-    TPRH_HIP(hipStreamSynchronize(cx.stream));
+    st = host_ctx_wait(cx);
+    if (st != TPRH_SUCCESS) return st;
     cx.dirty = false;
     if (g_timing) {
         float ms = -1.0f;

@@ -1,8 +1,10 @@
 // split.hip -- PTPR / TPR for systems larger than one workgroup.
 //
 // The reference needs every block of a system co-resident for its grid-wide
-// barriers and limits n to s^2 (src/ptpr.cu:543,581,655).  Here a system of m
-// slices is solved by three ordinary launches:
+// barriers and limits n to s^2 (src/ptpr.cu:543,581,655).  Here a batch is
+// solved in rounds of `chunk` systems, each round in one of two forms:
+//
+// Two passes (split_kernels.hpp), the library's form:
 //   K1 rows_kernel   : every lane group reduces its slices and writes two
 //                      boundary rows per slice (E_st before its update, E_ed);
 //   K2 reduced_kernel: one workgroup per system applies E_st += s1 E_ed
@@ -11,15 +13,26 @@
 //                      src/pcr.cpp; CR for TPR, src/cr.cpp);
 //   K3 stage3_kernel : every lane group reloads its slices, redoes stage 1
 //                      and runs stage 3 (src/ptpr.cpp:263-287 /
-//                      src/tpr.cpp:277-308).
+//                      src/tpr.cpp:277-308): the inputs are read twice,
+//                      36 B/eq of traffic.
 // K1 needs only the first and last row of every slice.  In PTPR's in-slice
 // PCR the final value of a slice's first (last) row depends only on the rows
 // at offsets 0 mod 2^p (-1 mod 2^p) of the previous levels, updated with the
 // same formulas in the same order as TPR's tree stage 1 (src/tpr.cpp:142-204
 // vs src/ptpr.cpp:140-182), so K1 runs the O(S) tree for both methods and its
-// rows are bit-identical to the full PCR's.  Nothing but those rows is
-// stored between the kernels: HBM traffic is 16 B/eq (K1) + 16 B/eq + 4 B/eq
-// (K3), with the stage-1 work of K1 a quarter of a full PTPR stage 1.
+// rows are bit-identical to the full PCR's.
+//
+// One pass (persistent_split_kernel), on request (split_form 2): each block
+// keeps its slices in registers from the load to the store and waits for
+// its system's reduced solve (20 B/eq).  It needs every block of a system
+// resident at once, and each wave of resident blocks exposes a reduced
+// solve's latency with nothing to overlap it: on MI355X config 3 (256 x
+// 65536, four waves of blocks) takes 0.25 ms this way against 0.11 ms in two
+// passes, so the library does not choose it by itself.
+//
+// Answers are the same bits in either form in exact arithmetic (the same
+// operations in the same order); in fast arithmetic the one-pass form runs
+// PTPR's two-level stage 1 on 16-lane slices, as the one-workgroup path does.
 #include <hip/hip_runtime.h>
 
 #include "split_kernels.hpp"
@@ -28,13 +41,19 @@ namespace tprh {
 
 namespace {
 
-template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
-void launch_round(const SplitArgs& p, size_t base, int grid, int threads, int lg_m, long long nslices, int cs, int m,
-                  int rthreads, size_t rshmem, hipStream_t stream) {
+// Clears the one-pass form's counters and flags (a kernel, not a memset, so
+// that a captured graph replays it correctly).
+__global__ void clear_words_kernel(int* __restrict__ w, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) w[i] = 0;
+}
+
+template <typename CT>
+BatchLoc<CT> round_loc(const SplitArgs& p, size_t base, int m) {
     const size_t cbase = p.shared_coef ? 0 : base;
     BatchLoc<CT> loc;
     loc.a = static_cast<const CT*>(p.a) + cbase;
-    loc.b = HAS_B ? static_cast<const CT*>(p.b) + cbase : nullptr;
+    loc.b = p.b ? static_cast<const CT*>(p.b) + cbase : nullptr;
     loc.c = static_cast<const CT*>(p.c) + cbase;
     loc.d = p.d + base;
     loc.x = p.x + base;
@@ -43,7 +62,18 @@ void launch_round(const SplitArgs& p, size_t base, int grid, int threads, int lg
     loc.n = p.n;
     loc.m = m;
     loc.cstride = p.shared_coef ? 0 : (size_t)p.n;
-    hipLaunchKernelGGL((rows_kernel<E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid), dim3(threads), 0, stream,
+    return loc;
+}
+
+// Two-pass round: K1 rows, K2 one workgroup per system for the reduced
+// system, K3 stage 3 (which reads the round's inputs a second time).
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+hipError_t launch_round_two_pass(const SplitArgs& p, size_t base, int cs, int m, size_t lds, hipStream_t stream) {
+    const BatchLoc<CT> loc = round_loc<CT>(p, base, m);
+    const int lg_m = __builtin_ctz((unsigned)m);
+    const long long nslices = (long long)cs * m;
+    const int grid = (int)((nslices * L + 255) / 256);
+    hipLaunchKernelGGL((rows_kernel<E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid), dim3(256), 0, stream,
                        loc, lg_m, nslices);
     // K2 touches only the rows and x_ed: one instantiation for every storage type
     BatchLoc<float> rl{};
@@ -51,44 +81,83 @@ void launch_round(const SplitArgs& p, size_t base, int grid, int threads, int lg
     rl.xed = p.xed;
     rl.n = p.n;
     rl.m = m;
-    hipLaunchKernelGGL((reduced_kernel<METHOD, BatchLoc<float>>), dim3(cs), dim3(rthreads), rshmem, stream, rl);
-    hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid), dim3(threads),
-                       0, stream, loc, lg_m, nslices);
+    int rthreads = m < 1024 ? m : 1024;
+    rthreads = (rthreads + 63) / 64 * 64;
+    hipLaunchKernelGGL((reduced_kernel<METHOD, BatchLoc<float>>), dim3(cs), dim3(rthreads), lds, stream, rl);
+    hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid), dim3(256), 0,
+                       stream, loc, lg_m, nslices);
+    return hipGetLastError();
+}
+
+// Workgroups of the one-pass kernel a device holds at once, with `lds`
+// bytes of LDS each (0 on a query error).
+template <typename K>
+int persistent_capacity(K kernel, size_t lds) {
+    int dev = 0, cus = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kPersistThreads, lds) != hipSuccess) return 0;
+    return cus * occ;
+}
+
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+hipError_t launch_round_persistent(const SplitArgs& p, size_t base, int cs, int m, int lg_bps, int capacity,
+                                   size_t lds, hipStream_t stream) {
+    const BatchLoc<CT> loc = round_loc<CT>(p, base, m);
+    const int words = kSplitSyncStride * (cs + 1);
+    hipLaunchKernelGGL(clear_words_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, p.sync, words);
+    const long long blocks = (long long)cs << lg_bps;
+    const int grid = (int)(blocks < capacity ? blocks : capacity);
+    hipLaunchKernelGGL((persistent_split_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid),
+                       dim3(kPersistThreads), lds, stream, loc, p.sync, cs, lg_bps);
+    return hipGetLastError();
+}
+
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+hipError_t launch_split_variant(const SplitArgs& p, int m, hipStream_t stream) {
+    // a system is bps blocks of 256 lanes (m L > 1024 on this path, so bps
+    // is a power of two >= 4)
+    const int bps = (int)(((long long)m * L) / kPersistThreads);
+    const int lg_bps = __builtin_ctz((unsigned)bps);
+    const size_t lds = sizeof(float) * 10 * (size_t)m;
+    const int lds_max = (int)(sizeof(float) * 10 * kSplitMaxM);
+    const auto pk = persistent_split_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>;
+    static std::atomic<unsigned long long> plds_set{0}, rlds_set{0};
+    hipError_t e = ensure_lds_limit(reinterpret_cast<const void*>(pk), lds_max, plds_set);
+    if (e == hipSuccess)
+        e = ensure_lds_limit(reinterpret_cast<const void*>(reduced_kernel<METHOD, BatchLoc<float>>), lds_max, rlds_set);
+    if (e != hipSuccess) return e;
+    // the one-pass form only when the caller asks for it (measured slower
+    // than two passes on MI355X wherever a round has more blocks than the
+    // chip holds at once: every wave of blocks waits out a reduced solve),
+    // and only where every block of a system can be resident at once
+    const int capacity = p.form == 2 ? persistent_capacity(pk, lds) : 0;
+    const bool one_pass = p.form == 2 && capacity >= bps;
+    for (int s0 = 0; s0 < p.nsys; s0 += p.chunk) {
+        const int cs = (p.nsys - s0 < p.chunk) ? p.nsys - s0 : p.chunk;
+        const size_t base = (size_t)s0 * p.n;
+        e = one_pass ? launch_round_persistent<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lg_bps, capacity,
+                                                                                     lds, stream)
+                     : launch_round_two_pass<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lds, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <int METHOD, int E, int L, int MODE>
 hipError_t launch_split_el(const SplitArgs& p, hipStream_t stream) {
     const int m = p.n / (E * L);
-    const int lg_m = __builtin_ctz((unsigned)m);
-    const int threads = 256;
-    const int slices_per_block = threads / L;
-    static std::atomic<unsigned long long> lds_set{0};  // K2 keeps 10 rows of m floats in LDS
-    hipError_t le = ensure_lds_limit(reinterpret_cast<const void*>(reduced_kernel<METHOD, BatchLoc<float>>),
-                                     (int)(sizeof(float) * 10 * kSplitMaxM), lds_set);
-    if (le != hipSuccess) return le;
-    int rthreads = m < 1024 ? m : 1024;
-    rthreads = (rthreads + 63) / 64 * 64;
-    const size_t rshmem = sizeof(float) * 10 * (size_t)m;
-    for (int s0 = 0; s0 < p.nsys; s0 += p.chunk) {
-        const int cs = (p.nsys - s0 < p.chunk) ? p.nsys - s0 : p.chunk;
-        const size_t base = (size_t)s0 * p.n;
-        const long long nslices = (long long)cs * m;
-        const int grid = (int)((nslices + slices_per_block - 1) / slices_per_block);
-        const bool hb = p.b != nullptr;
-        if (p.f16) {
-            if (hb) launch_round<METHOD, E, L, MODE, true, _Float16, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
-            else launch_round<METHOD, E, L, MODE, false, _Float16, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
-        } else if (p.refine) {
-            if (hb) launch_round<METHOD, E, L, MODE, true, float, true>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
-            else launch_round<METHOD, E, L, MODE, false, float, true>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
-        } else {
-            if (hb) launch_round<METHOD, E, L, MODE, true, float, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
-            else launch_round<METHOD, E, L, MODE, false, float, false>(p, base, grid, threads, lg_m, nslices, cs, m, rthreads, rshmem, stream);
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+    const bool hb = p.b != nullptr;
+    if (p.f16) {
+        return hb ? launch_split_variant<METHOD, E, L, MODE, true, _Float16, false>(p, m, stream)
+                  : launch_split_variant<METHOD, E, L, MODE, false, _Float16, false>(p, m, stream);
     }
-    return hipSuccess;
+    if (p.refine) {
+        return hb ? launch_split_variant<METHOD, E, L, MODE, true, float, true>(p, m, stream)
+                  : launch_split_variant<METHOD, E, L, MODE, false, float, true>(p, m, stream);
+    }
+    return hb ? launch_split_variant<METHOD, E, L, MODE, true, float, false>(p, m, stream)
+              : launch_split_variant<METHOD, E, L, MODE, false, float, false>(p, m, stream);
 }
 
 template <int METHOD, int MODE>

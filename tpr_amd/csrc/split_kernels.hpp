@@ -84,10 +84,29 @@ __global__ __launch_bounds__(256) void rows_kernel(Loc loc, int lg_m, long long 
     }
 }
 
-template <int METHOD, typename Loc>
-__global__ __launch_bounds__(1024) void reduced_kernel(Loc loc) {
-    extern __shared__ float lds[];
-    const int m = loc.m;
+// The reduced system of one system from its m slices' boundary rows
+// (sysrows, kRowWords floats per slice): E_st += s1 E_ed
+// (src/ptpr.cpp:193-211, src/tpr.cpp:327-340), the reduced rows
+// (src/ptpr.cpp:218-241) and their solve in LDS, PCR for PTPR
+// (src/pcr.cpp:15-89) or CR for TPR (src/cr.cpp:23-116), by the nthr threads
+// t = 0 .. nthr-1 of one workgroup; x at every slice end goes to xed.
+// lds: 10 m floats.  Every thread of the workgroup must call it.
+// Loads and stores of the data blocks exchange inside one launch (boundary
+// rows, x at slice ends, counters, flags): atomic at device scope, so that
+// they are coherent across the chip's L2 caches without writing back a whole
+// L2 (which a device-scope release fence does, and which costs far more than
+// these few words).
+__device__ __forceinline__ float ld_dev(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int METHOD, bool COHERENT = false>
+__device__ __forceinline__ void reduced_rows_solve(const float* sysrows, float* xed, int m, float* lds, int t,
+                                                   int nthr) {
+    auto ld = [](const float* p) { return COHERENT ? ld_dev(p) : *p; };
     float* st_a = lds;
     float* st_c = st_a + m;
     float* st_r = st_c + m;
@@ -98,22 +117,19 @@ __global__ __launch_bounds__(1024) void reduced_kernel(Loc loc) {
     float* rc1 = ra1 + m;
     float* rr1 = rc1 + m;
     float* xo = rr1 + m;
-    const float* sysrows = loc.Rows(blockIdx.x);
-    const int nthr = blockDim.x;
-    for (int j = threadIdx.x; j < m; j += nthr) {
-        // E_st += s1 * E_ed, src/ptpr.cpp:193-211 (src/tpr.cpp:327-340)
+    for (int j = t; j < m; j += nthr) {
         const float* r = sysrows + (size_t)j * kRowWords;
-        const float c_ed = r[4];
-        const float s1 = (fabsf(c_ed) < FLT_EPSILON) ? -1.0f : -r[1] / c_ed;
-        st_a[j] = r[0] + s1 * r[3];
+        const float c_ed = ld(r + 4);
+        const float s1 = (fabsf(c_ed) < FLT_EPSILON) ? -1.0f : -ld(r + 1) / c_ed;
+        st_a[j] = ld(r + 0) + s1 * ld(r + 3);
         st_c[j] = s1;
-        st_r[j] = r[2] + s1 * r[5];
+        st_r[j] = ld(r + 2) + s1 * ld(r + 5);
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < m; j += nthr) {
-        // reduced row j from (E_ed(j), E_st(j+1)), src/ptpr.cpp:218-241
+    for (int j = t; j < m; j += nthr) {
+        // reduced row j from (E_ed(j), E_st(j+1))
         const float* r = sysrows + (size_t)j * kRowWords;
-        const float ak = r[3], ck = r[4], rk = r[5];
+        const float ak = ld(r + 3), ck = ld(r + 4), rk = ld(r + 5);
         if (j < m - 1) {
             const float akr = st_a[j + 1], ckr = st_c[j + 1], rkr = st_r[j + 1];
             const float inv = 1.0f / (1.0f - akr * ck);
@@ -129,13 +145,21 @@ __global__ __launch_bounds__(1024) void reduced_kernel(Loc loc) {
     __syncthreads();
     const float* res;
     if constexpr (METHOD == 0) {
-        res = lds_pcr(ra0, rc0, rr0, ra1, rc1, rr1, m, threadIdx.x, nthr);
+        res = lds_pcr(ra0, rc0, rr0, ra1, rc1, rr1, m, t, nthr);
     } else {
-        lds_cr(ra0, rc0, rr0, xo, m, threadIdx.x, nthr);
+        lds_cr(ra0, rc0, rr0, xo, m, t, nthr);
         res = xo;
     }
-    float* xed = loc.Xed(blockIdx.x);
-    for (int j = threadIdx.x; j < m; j += nthr) xed[j] = res[j];
+    for (int j = t; j < m; j += nthr) {
+        if constexpr (COHERENT) st_dev(xed + j, res[j]);
+        else xed[j] = res[j];
+    }
+}
+
+template <int METHOD, typename Loc>
+__global__ __launch_bounds__(1024) void reduced_kernel(Loc loc) {
+    extern __shared__ float lds[];
+    reduced_rows_solve<METHOD>(loc.Rows(blockIdx.x), loc.Xed(blockIdx.x), loc.m, lds, threadIdx.x, blockDim.x);
 }
 
 template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
@@ -192,6 +216,167 @@ __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long lon
         for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
     }
     if (valid) store_e<E>(x + lo, X);
+}
+
+// ---------------------------------------------------------------------------
+// One-pass persistent form of the multi-workgroup path: every equation is
+// read once and x written once (20 B/eq), against 36 B/eq for the three
+// kernels above, whose stage 3 reloads the inputs to redo stage 1.
+//
+// A block is kPersistThreads lanes = kPersistThreads / L consecutive slices
+// of one system; a system of m slices is bps = m L / kPersistThreads blocks.
+// Each workgroup of the persistent grid takes tickets t = 0, 1, ... from an
+// atomic counter (block t % bps of system t / bps) until they run out, and
+// for each block
+//   1. loads its slices (vector loads, the division by the diagonal in
+//      registers) and runs stage 1 in registers;
+//   2. writes each slice's boundary rows (E_st before its update, E_ed) to
+//      the workspace and counts itself in the system's arrival counter;
+//   3. the last block of the system to arrive solves the reduced system
+//      (reduced_rows_solve) and raises the system's flag; the other blocks
+//      wait for the flag with their slices still in registers;
+//   4. runs stage 3 in registers from x at the slice ends and stores x.
+// The wait cannot block forever: tickets are handed out in system order and
+// only to workgroups already running, so the oldest unfinished system has
+// all of its blocks among the running workgroups as soon as at least bps of
+// them are resident (the host launches this form only when the device holds
+// that many at once), and its last arriver finishes it.  The wait is also
+// bounded (about two seconds) and records a timeout in sync[] if it ever
+// expires, so that no workgroup spins for ever.
+//
+// sync[] (cleared before every launch), one 128-byte line per system so
+// that waiting workgroups of different systems poll different lines:
+// sync[0] ticket counter, sync[1] wait timeouts, sync[kSyncStride (q + 1)]
+// arrivals of system q, sync[kSyncStride (q + 1) + kSyncStride / 2] its flag.
+constexpr int kPersistThreads = 256;
+constexpr long long kPersistSpinLimit = 1LL << 23;
+
+__device__ __forceinline__ bool wait_flag(int* flag) {
+    for (long long i = 0; i < kPersistSpinLimit; i++) {
+        if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        __builtin_amdgcn_s_sleep(8);  // about 0.2 us between polls
+    }
+    return false;
+}
+
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
+__global__ __launch_bounds__(kPersistThreads) void persistent_split_kernel(Loc loc, int* __restrict__ sync, int nsys,
+                                                                           int lg_bps) {
+    extern __shared__ float lds[];  // 10 m floats: the reduced solve
+    __shared__ int s_ticket, s_last;
+    constexpr int kSlices = kPersistThreads / L;
+    const int m = loc.m;
+    const int bps = 1 << lg_bps;
+    const int total = nsys << lg_bps;
+    const int l = threadIdx.x % L;
+    const int jb = threadIdx.x / L;  // slice inside the block
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(&sync[0], 1);
+        __syncthreads();
+        const int tk = s_ticket;
+        __syncthreads();  // every thread has read the ticket before it is rewritten
+        if (tk >= total) break;  // uniform: the workgroup leaves once the tickets are gone
+        const int q = tk >> lg_bps;
+        const int j = (tk & (bps - 1)) * kSlices + jb;  // slice inside the system
+        const size_t lo = (size_t)j * (E * L) + (size_t)l * E;
+        RawPart<E> raw;
+        load_raw_t<E, CT>(loc.A(q), HAS_B ? loc.B(q) : nullptr, loc.C(q), loc.D(q), lo, lo, raw);
+        float xo[E];
+        if constexpr (REFINE) {
+            // every block of the system loads its x (and the halo from its
+            // neighbours) before the system's last arriver can start the
+            // reduced solve, and no x is stored before that solve is done
+            float xl, xr;
+            load_x_with_halo<E>(loc.X(q), 0, lo, loc.n, xo, xl, xr);
+            residual_fp64<E, HAS_B>(xo, xl, xr, raw);
+        }
+        float* row = loc.Rows(q) + (size_t)j * kRowWords;
+        // stage 1, and the slice's boundary rows
+        float A[E], C[E], R[E];
+        float Ao[E], Co[E], Ro[E];
+        Fast16State<(E >= 2 ? E : 2)> fs;
+        constexpr bool kTwoLevel = METHOD == 0 && MODE == kFast && L == 16 && E >= 2;
+        float st_a, st_c, st_r, ed_a, ed_c, ed_r;
+        if constexpr (kTwoLevel) {
+            fast16_stage1<E>(raw, HAS_B, l, fs);
+            st_a = fs.sA, st_c = fs.sC, st_r = fs.sR;
+            ed_a = fs.eA, ed_c = fs.eC, ed_r = fs.eR;
+        } else {
+            normalise<E, MODE>(raw, HAS_B, A, C, R);
+            if constexpr (METHOD == 0) {
+                ptpr_stage1<E, L, MODE>(A, C, R, l);
+            } else {
+#pragma unroll
+                for (int i = 0; i < E; i++) {
+                    Ao[i] = A[i];
+                    Co[i] = C[i];
+                    Ro[i] = R[i];
+                }
+                tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+            }
+            st_a = A[0], st_c = C[0], st_r = R[0];
+            ed_a = A[E - 1], ed_c = C[E - 1], ed_r = R[E - 1];
+        }
+        if (l == 0) {
+            st_dev(row + 0, st_a);
+            st_dev(row + 1, st_c);
+            st_dev(row + 2, st_r);
+        }
+        if (l == L - 1) {
+            st_dev(row + 3, ed_a);
+            st_dev(row + 4, ed_c);
+            st_dev(row + 5, ed_r);
+        }
+        // publish: every wave's row stores have completed (device-coherent
+        // stores, acknowledged once they are visible to the whole chip)
+        // before the arrival is counted; the count, the flag and every read
+        // of rows and x_ed below are device-coherent too, and each read is
+        // issued only after the count or the flag it depends on was seen
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        int* sys_sync = sync + (size_t)kSplitSyncStride * (q + 1);
+        if (threadIdx.x == 0)
+            s_last = __hip_atomic_fetch_add(sys_sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bps - 1;
+        __syncthreads();
+        if (s_last) {
+            reduced_rows_solve<METHOD, true>(loc.Rows(q), loc.Xed(q), m, lds, threadIdx.x, kPersistThreads);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(sys_sync + kSplitSyncStride / 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (threadIdx.x == 0 && !wait_flag(sys_sync + kSplitSyncStride / 2))
+                __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+        }
+        // stage 3
+        const float* xed = loc.Xed(q);
+        const float xe = ld_dev(xed + j);
+        const float xprev = (j > 0) ? ld_dev(xed + j - 1) : 0.0f;
+        float X[E];
+        if constexpr (METHOD == 0) {
+            // the slice's last row (E_ed) from its last lane
+            const int src = (int)(threadIdx.x & ~(unsigned)(L - 1)) + (L - 1);
+            SliceEnd se;
+            se.a = __shfl(ed_a, src, 64);
+            se.c = __shfl(ed_c, src, 64);
+            se.r = __shfl(ed_r, src, 64);
+            if constexpr (kTwoLevel) {
+                fast16_stage3<E>(fs, xprev, xe, se, l, X);
+            } else {
+                ptpr_stage3<E, L, MODE>(A, C, R, xprev, xe, se, l, X);
+            }
+        } else {
+            tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xe, l, X);
+        }
+        if constexpr (REFINE) {
+#pragma unroll
+            for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
+        }
+        store_e<E>(loc.X(q) + lo, X);
+        // the next ticket's stage 1 may reuse lds only after this block's
+        // reduced solve is complete (it is: the flag came after it)
+    }
 }
 
 }  // namespace tprh
