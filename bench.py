@@ -29,6 +29,18 @@ rocSPARSE comparator, a device copy (the practical HBM ceiling) and
 "batch_stream": the same batches issued over 2 and 3 streams so that
 consecutive launches overlap (one batch's compute tail under the next one's
 loads), over eight buffer sets.
+
+The rest of the metric, from short runs of the other configurations
+(--no-side leaves them out):
+  "serve"      the serving loop, Poisson arrivals at 2000 req/s for 2 s and
+               4000 requests closed-loop (p50/p99/max latency, systems/s,
+               against one launch per right-hand side); "latency_ms" at the
+               top level is its Poisson latency;
+  "uneven"     20000 systems of N = 2^U{8..14} in one persistent launch
+               (ms, GB/s, fraction of the HBM peak; accuracy of 200 systems);
+  "precision"  256 weakly dominant systems x N=65536: fp32, fp16 storage + one
+               FP64-residual refinement step, fp32 + one step (ms, errors per
+               system against the host FP64 Thomas solve).
 """
 import argparse
 import ctypes
@@ -63,6 +75,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
+    ap.add_argument("--split-form", type=int, default=0, choices=[0, 1, 2],
+                    help="precision workload: form of the multi-workgroup path (0 the library's choice, 1 two "
+                         "passes, 2 one pass)")
     ap.add_argument("--fp32-only", action="store_true",
                     help="precision workload: the fp32 solve alone (no fp16, refinement or per-system variants), "
                          "e.g. for counter runs")
@@ -77,6 +92,9 @@ def parse():
     ap.add_argument("--seconds", type=float, default=5.0, help="serve workload: length of the Poisson run")
     ap.add_argument("--retune", action="store_true",
                     help="uneven workload: run the autotuner now instead of loading the committed schedule")
+    ap.add_argument("--no-side", action="store_true",
+                    help="batch workload: leave out the short config-2/3/4 legs (uneven, precision, serve) that "
+                         "the default line carries beside the headline")
     return ap.parse_args()
 
 
@@ -374,6 +392,113 @@ def copy_workload(args):
                       "frac_of_peak": round(torch_gbps / HBM_PEAK_GBS, 4)}))
 
 
+PREC_NSYS, PREC_N, PREC_S = 256, 65536, 256
+
+
+def graph_ms(fn, reps):
+    """Device time per call of fn: `reps` calls captured in one HIP graph,
+    replayed once untimed and once between HIP events on the current
+    stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(reps):
+            fn()
+    g.replay()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def precision_setup(chunk=0, split_form=0):
+    """Config 3's batch: 256 weakly dominant fp32 systems x N=65536 (and its
+    fp16-stored coefficients), generated on the device, with a workspace for
+    the multi-workgroup path."""
+    import torch
+
+    from tpr_amd import batched as B
+    nsys, n, s = PREC_NSYS, PREC_N, PREC_S
+    t = B.alloc_batch(nsys, n)
+    B.generate("weak", 30303, 0, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
+    h16 = {k: t[k].half() for k in "abc"}
+    ws = torch.empty(B.workspace_bytes(B.make_desc(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, chunk=chunk)),
+                     dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    return {"t": t, "h16": h16, "ws": ws, "chunk": chunk, "form": split_form}
+
+
+def precision_variants(P, combos, reps, method="ptpr"):
+    """Times each (fp16 coefficients?, refinement steps) variant and keeps
+    its answers; algorithmic bytes: solve 20 B/eq (14 with fp16 a, b, c),
+    each refinement step 24 B/eq (a, b, c, d, x read; x written)."""
+    from tpr_amd import batched as B
+    nsys, n, s = PREC_NSYS, PREC_N, PREC_S
+    t, h16, ws, ch, form = P["t"], P["h16"], P["ws"], P["chunk"], P["form"]
+
+    def solve(coef16, refine_steps):
+        src = h16 if coef16 else t
+        B.solve_ex(src["a"], src["b"], src["c"], t["d"], t["x"], n, s, method=method, workspace=ws, chunk=ch,
+                   split_form=form)
+        for _ in range(refine_steps):
+            B.solve_ex(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method=method, refine=True, workspace=ws,
+                       chunk=ch, split_form=form)
+
+    import torch
+    out = []
+    for coef16, steps in combos:
+        ms = graph_ms(lambda: solve(coef16, steps), reps)
+        solve(coef16, steps)
+        torch.cuda.synchronize()
+        alg = nsys * n * ((14 if coef16 else 20) + 24 * steps)
+        out.append({"name": "%s%s" % ("fp16" if coef16 else "fp32", "+%d refine" % steps if steps else ""),
+                    "ms": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
+                    "GBps": round(alg / (ms / 1e3) / 1e9, 1), "x": t["x"].cpu().numpy()})
+    return out
+
+
+def precision_errors(P, variants):
+    """Each variant's error per system against the host FP64 Thomas solve
+    of the fp32 system (the oracle's, on this host)."""
+    import numpy as np
+
+    import oracle as O
+    t, n = P["t"], PREC_N
+    host = {k: t[k].cpu().numpy() for k in "abcd"}
+    xf = O.thomas_f64_batch(host["a"], host["b"], host["c"], host["d"], n).reshape(PREC_NSYS, n)
+    for v in variants:
+        if v.get("x") is None:
+            continue
+        x = v["x"].reshape(PREC_NSYS, n).astype(np.float64)
+        err = np.linalg.norm(x - xf, axis=1) / np.linalg.norm(xf, axis=1)
+        v["rel_l2_vs_fp64_thomas_max"] = float(err.max())
+        v["rel_l2_vs_fp64_thomas_median"] = float(np.median(err))
+    return host, xf
+
+
+def precision_side(args):
+    """Config 3 for the default line: fp32 solve, fp16-stored coefficients
+    followed by one FP64-residual refinement step, and fp32 + one step;
+    10 graph-replayed calls each; errors per system vs FP64 Thomas."""
+    P = precision_setup()
+    vs = precision_variants(P, [(False, 0), (True, 1), (False, 1)], 10)
+    if not args.no_cpu:
+        precision_errors(P, vs)
+    for v in vs:
+        v.pop("x", None)
+    head = vs[0]
+    return {"workload": "%d weakly dominant fp32 systems x N=%d, S=%d, multi-workgroup path" %
+                        (PREC_NSYS, PREC_N, PREC_S),
+            "ms": head["ms"], "systems_per_s": head["systems_per_s"], "GBps": head["GBps"],
+            "frac": round(head["GBps"] / HBM_PEAK_GBS, 4), "variants": vs}
+
+
 def precision_workload(args):
     """256 weakly dominant systems x N=65536, S=256 (m = 256 slices: the
     multi-workgroup path).  Variants: fp32 PTPR, fp16-stored coefficients,
@@ -384,67 +509,26 @@ def precision_workload(args):
     import numpy as np
     import torch
 
-    import tpr_amd
     from tpr_amd import batched as B
 
-    nsys, n, s = 256, 65536, 256
+    nsys, n, s = PREC_NSYS, PREC_N, PREC_S
     torch.cuda.set_device(0)
-    t = B.alloc_batch(nsys, n)
-    B.generate("weak", 30303, 0, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
-    h16 = {k: t[k].half() for k in "abc"}
-    torch.cuda.synchronize()
     ch = args.chunk
-    ws = torch.empty(B.workspace_bytes(B.make_desc(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, chunk=ch)),
-                     dtype=torch.uint8, device="cuda")
-
-    def solve(coef16, refine_steps, method="ptpr"):
-        src = h16 if coef16 else t
-        B.solve_ex(src["a"], src["b"], src["c"], t["d"], t["x"], n, s, method=method, workspace=ws, chunk=ch)
-        for _ in range(refine_steps):
-            B.solve_ex(t["a"], t["b"], t["c"], t["d"], t["x"], n, s, method=method, refine=True, workspace=ws,
-                       chunk=ch)
+    P = precision_setup(ch, args.split_form)
+    t, ws = P["t"], P["ws"]
 
     def per_system():
         for k in range(nsys):
             sl = slice(k * n, (k + 1) * n)
             B.solve_ex(t["a"][sl], t["b"][sl], t["c"][sl], t["d"][sl], t["x"][sl], n, s, workspace=ws)
 
-    def timed(fn, reps):
-        fn()
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            for _ in range(reps):
-                fn()
-        g.replay()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        g.replay()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
-
-    variants = []
     combos = [(False, 0)] if args.fp32_only else [(c16, st) for c16 in (False, True) for st in (0, 1, 2)]
-    for coef16, steps in combos:
-        ms = timed(lambda: solve(coef16, steps), 10)
-        solve(coef16, steps)
-        torch.cuda.synchronize()
-        x = t["x"].cpu().numpy()
-        # algorithmic bytes: solve 20 B/eq (14 with fp16 a, b, c), each
-        # refinement step 24 B/eq (a, b, c, d, x read; x written)
-        alg = nsys * n * ((14 if coef16 else 20) + 24 * steps)
-        variants.append({"name": "%s%s" % ("fp16" if coef16 else "fp32", "+%d refine" % steps if steps else ""),
-                         "ms": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
-                         "GBps": round(alg / (ms / 1e3) / 1e9, 1), "x": x})
+    variants = precision_variants(P, combos, 10)
     if not args.fp32_only:
-        ms = timed(per_system, 2)
+        ms = graph_ms(per_system, 2)
         variants.append({"name": "fp32, one system per launch", "ms": round(ms, 4),
                          "systems_per_s": round(nsys / (ms / 1e3), 1),
                          "GBps": round(20 * nsys * n / (ms / 1e3) / 1e9, 1), "x": None})
-    host = {k: t[k].cpu().numpy() for k in ("a", "b", "c", "d")}
     head = variants[0]  # the fp32 solve is the line's value; the rest are variants of it
     out = {
         "metric": METRIC, "value": head["systems_per_s"], "unit": "systems/s", "n_gpus": 1, "steps": 10,
@@ -452,30 +536,21 @@ def precision_workload(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "%d weakly dominant fp32 systems x N=%d, S=%d (b = |a|+|c|+eps, eps ~ U[1e-4,1e-2]), "
                                "multi-workgroup path, one step = one batch solve" % (nsys, n, s),
-                   "method": "ptpr", "arith": "fast", "chunk": ch},
-        # algorithmic bytes 20 B/eq over the three kernels of a round (K1 rows,
-        # K2 reduced systems, K3 stage 3), per batch
+                   "method": "ptpr", "arith": "fast", "chunk": ch, "split_form": args.split_form},
+        # algorithmic bytes 20 B/eq over the kernels of a solve, per batch
         "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
         "variants": variants,
     }
-    # bytes past L2 per fp32 solve (the three kernels) from FETCH_SIZE /
-    # WRITE_SIZE runs, scaled by the copy calibration: the stage-3 re-read
-    # makes it ~1.8x the algorithmic bytes
+    # bytes past L2 per fp32 solve from FETCH_SIZE / WRITE_SIZE runs, scaled
+    # by the copy calibration
     tf = os.path.join(ROOT, "profiles", "traffic_ptpr_split256x65536.json")
     if ch == 0 and os.path.exists(tf):
         with open(tf) as f:
             out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_step")
     if not args.no_cpu:
         import oracle as O
-        xf = O.thomas_f64_batch(host["a"], host["b"], host["c"], host["d"], n).reshape(nsys, n)
-        for v in variants:
-            if v["x"] is None:
-                continue
-            x = v["x"].reshape(nsys, n).astype(np.float64)
-            err = np.linalg.norm(x - xf, axis=1) / np.linalg.norm(xf, axis=1)
-            v["rel_l2_vs_fp64_thomas_max"] = float(err.max())
-            v["rel_l2_vs_fp64_thomas_median"] = float(np.median(err))
+        host, xf = precision_errors(P, variants)
         kind = "reference" if O.ref_available() else "port"
         solvef = O.ref_solve_batch if kind == "reference" else O.solve_batch
         k = 32
@@ -491,6 +566,94 @@ def precision_workload(args):
     print(json.dumps(out))
 
 
+UNEVEN_NSYS = 20000
+
+
+def uneven_setup(args):
+    """Config 2's batch: 20000 systems of N = 2^U{8..14} (log-uniform over
+    the powers of two), diagonally dominant generator, in one uneven-batch
+    layout; plans for the committed autotuned schedule and for the uniform
+    s = min(N, 256) schedule."""
+    import torch
+
+    from tpr_amd import autotune
+    from tpr_amd.uneven import UnevenPlan, log_uniform_sizes
+
+    sizes = log_uniform_sizes(UNEVEN_NSYS, 8, 14, seed=2020)
+    if args.retune or not os.path.exists(autotune.DEFAULT_PATH):
+        table = autotune.tune(method=args.method)
+        tuned_from = "tuned in this run"
+    else:
+        table = autotune.load()
+        tuned_from = os.path.relpath(autotune.DEFAULT_PATH, ROOT)
+    sched = autotune.schedule_of(table)
+    plans = {"autotuned": UnevenPlan(sizes, args.method, args.arith, sched),
+             "uniform s=256 schedule": UnevenPlan(sizes, args.method, args.arith, None)}
+    tot = plans["autotuned"].equations
+    a, b, c, d, x, xt = (torch.empty(tot, device="cuda") for _ in range(6))
+    plans["autotuned"].generate("dominant", 4242, 0, a, b, c, d, xt)
+    torch.cuda.synchronize()
+    return {"sizes": sizes, "sched": sched, "tuned_from": tuned_from, "plans": plans, "tot": tot,
+            "arr": {"a": a, "b": b, "c": c, "d": d, "x": x, "xt": xt}}
+
+
+def uneven_time(U, runs, steps):
+    """Per-step device time of each run (graph of `steps` calls), as GB/s at
+    20 B per equation and as a fraction of the HBM peak."""
+    res = {}
+    for name, fn in runs.items():
+        ms = graph_ms(fn, steps)
+        gbps = 20.0 * U["tot"] / (ms / 1e3) / 1e9
+        res[name] = {"ms_per_step": round(ms, 4), "systems_per_s": round(UNEVEN_NSYS / (ms / 1e3), 1),
+                     "GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
+    return res
+
+
+def uneven_accuracy(args, U, count):
+    """The autotuned answers of `count` systems drawn at random, against the
+    oracle at the same s per class and against the generator's truth."""
+    import numpy as np
+    import torch
+
+    import oracle as O
+    p = U["plans"]["autotuned"]
+    A = U["arr"]
+    p.solve(A["a"], A["b"], A["c"], A["d"], A["x"])
+    torch.cuda.synchronize()
+    h = {k: v.cpu().numpy() for k, v in A.items()}
+    pick = np.random.default_rng(7).choice(UNEVEN_NSYS, count, replace=False)
+    worst_o = worst_t = 0.0
+    for k in pick:
+        lo, hi = p.offset(int(k)), p.offset(int(k) + 1)
+        n = int(U["sizes"][k])
+        ref = O.solve_batch(args.method, h["a"][lo:hi], h["b"][lo:hi], h["c"][lo:hi], h["d"][lo:hi], n,
+                            U["sched"].get(n, (min(n, 256),))[0])
+        worst_o = max(worst_o, O.rel_l2(h["x"][lo:hi], ref))
+        worst_t = max(worst_t, O.rel_l2(h["x"][lo:hi], h["xt"][lo:hi]))
+    return {"systems_checked": int(count), "max_rel_l2_vs_oracle_same_s": worst_o, "max_rel_l2_vs_truth": worst_t,
+            "tolerance": 1e-5}
+
+
+def uneven_side(args):
+    """Config 2 for the default line: the persistent launch with the
+    autotuned and the uniform schedules, 10 graph-replayed steps each;
+    accuracy of 200 systems against the oracle."""
+    U = uneven_setup(args)
+    A = U["arr"]
+    runs = {name: (lambda p=p: p.solve(A["a"], A["b"], A["c"], A["d"], A["x"])) for name, p in U["plans"].items()}
+    res = uneven_time(U, runs, 10)
+    best = res["autotuned"]
+    out = {"workload": "%d fp32 systems, N = 2^U{8..14} (%d equations), one persistent launch" %
+                       (UNEVEN_NSYS, U["tot"]),
+           "ms": best["ms_per_step"], "systems_per_s": best["systems_per_s"], "GBps": best["GBps"],
+           "frac": best["hbm_frac"], "schedules": res, "schedule_source": U["tuned_from"]}
+    if not args.no_cpu:
+        out["accuracy"] = uneven_accuracy(args, U, 200)
+    for p in U["plans"].values():
+        p.close()
+    return out
+
+
 def uneven_workload(args):
     """20000 systems, N = 2^U{8..14} (log-uniform over the powers of two),
     diagonally dominant generator, one persistent launch per step, replayed
@@ -504,32 +667,20 @@ def uneven_workload(args):
     import numpy as np
     import torch
 
-    from tpr_amd import autotune
     from tpr_amd import batched as B
-    from tpr_amd.uneven import UnevenPlan, log_uniform_sizes
+    from tpr_amd.uneven import UnevenPlan
 
-    nsys = 20000
     torch.cuda.set_device(0)
-    sizes = log_uniform_sizes(nsys, 8, 14, seed=2020)
-    if args.retune or not os.path.exists(autotune.DEFAULT_PATH):
-        table = autotune.tune(method=args.method)
-        tuned_from = "tuned in this run"
-    else:
-        table = autotune.load()
-        tuned_from = os.path.relpath(autotune.DEFAULT_PATH, ROOT)
-    sched = autotune.schedule_of(table)
-    plans = {"autotuned": UnevenPlan(sizes, args.method, args.arith, sched),
-             "uniform s=256 schedule": UnevenPlan(sizes, args.method, args.arith, None)}
-    tot = plans["autotuned"].equations
-    a, b, c, d, x, xt = (torch.empty(tot, device="cuda") for _ in range(6))
-    plans["autotuned"].generate("dominant", 4242, 0, a, b, c, d, xt)
+    U = uneven_setup(args)
+    sizes, sched, A, tot = U["sizes"], U["sched"], U["arr"], U["tot"]
+    nsys = UNEVEN_NSYS
     # class-sorted copy for the grouped launches (stable order inside a class)
     order = np.argsort(sizes, kind="stable")
     srt = UnevenPlan(sizes[order], args.method, args.arith, None)
-    idx = np.concatenate([np.arange(plans["autotuned"].offset(k), plans["autotuned"].offset(k + 1))
-                          for k in order])
+    pa = U["plans"]["autotuned"]
+    idx = np.concatenate([np.arange(pa.offset(k), pa.offset(k + 1)) for k in order])
     idx_t = torch.from_numpy(idx).cuda()
-    sa, sb, sc, sd = (v[idx_t].contiguous() for v in (a, b, c, d))
+    sa, sb, sc, sd = (A[v][idx_t].contiguous() for v in "abcd")
     sx = torch.empty_like(sa)
     groups = []
     for n in sorted(set(sizes.tolist())):
@@ -543,34 +694,10 @@ def uneven_workload(args):
             B.solve_batched(sa[lo:hi], sb[lo:hi], sc[lo:hi], sd[lo:hi], sx[lo:hi], n, min(n, 256),
                             method=args.method, arith=args.arith)
 
-    runs = {name: (lambda p=p: p.solve(a, b, c, d, x)) for name, p in plans.items()}
+    runs = {name: (lambda p=p: p.solve(A["a"], A["b"], A["c"], A["d"], A["x"])) for name, p in U["plans"].items()}
     runs["grouped launches"] = grouped
     steps = args.steps
-    res = {}
-    for name, fn in runs.items():
-        for _ in range(max(1, args.warmup)):
-            fn()
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            for _ in range(steps):
-                fn()
-        g.replay()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        g.replay()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
-        gbps = 20.0 * tot / (ms / 1e3) / 1e9
-        res[name] = {"ms_per_step": round(ms, 4), "systems_per_s": round(nsys / (ms / 1e3), 1),
-                     "GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
-    # accuracy: the autotuned answers against the oracle (same s per class)
-    # and against the truth, on a sample of systems
-    plans["autotuned"].solve(a, b, c, d, x)
-    torch.cuda.synchronize()
-    h = {k: v.cpu().numpy() for k, v in (("a", a), ("b", b), ("c", c), ("d", d), ("x", x), ("xt", xt))}
+    res = uneven_time(U, runs, steps)
     best = res["autotuned"]
     out = {
         "metric": METRIC, "value": best["systems_per_s"], "unit": "systems/s", "n_gpus": 1, "steps": steps,
@@ -579,7 +706,7 @@ def uneven_workload(args):
         "config": {"workload": "uneven batch: %d fp32 systems, N = 2^U{8..14} (%d equations), diagonally dominant "
                                "generator, one persistent launch per step replayed from a HIP graph" % (nsys, tot),
                    "method": args.method, "arith": args.arith, "schedule": {str(k): v for k, v in sched.items()},
-                   "schedule_source": tuned_from,
+                   "schedule_source": U["tuned_from"],
                    "class_counts": {str(n): int((sizes == n).sum()) for n in sorted(set(sizes.tolist()))}},
         "roofline": {"bound": "hbm", "achieved": best["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": best["hbm_frac"], "traffic": None},
@@ -593,19 +720,7 @@ def uneven_workload(args):
             out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
     if not args.no_cpu:
         import oracle as O
-        pa = plans["autotuned"]
-        rng = np.random.default_rng(7)
-        pick = rng.choice(nsys, 400, replace=False)
-        worst_o = worst_t = 0.0
-        for k in pick:
-            lo, hi = pa.offset(int(k)), pa.offset(int(k) + 1)
-            n = int(sizes[k])
-            ref = O.solve_batch(args.method, h["a"][lo:hi], h["b"][lo:hi], h["c"][lo:hi], h["d"][lo:hi], n,
-                                sched.get(n, (min(n, 256),))[0])
-            worst_o = max(worst_o, O.rel_l2(h["x"][lo:hi], ref))
-            worst_t = max(worst_t, O.rel_l2(h["x"][lo:hi], h["xt"][lo:hi]))
-        out["accuracy"] = {"systems_checked": len(pick), "max_rel_l2_vs_oracle_same_s": worst_o,
-                           "max_rel_l2_vs_truth": worst_t, "tolerance": 1e-5}
+        out["accuracy"] = uneven_accuracy(args, U, 400)
         kind = "reference" if O.ref_available() else "port"
         solvef = O.ref_solve_batch if kind == "reference" else O.solve_batch
         sample = []
@@ -632,34 +747,26 @@ def uneven_workload(args):
     print(json.dumps(out))
 
 
-def serve_workload(args):
-    """The serving loop under load, driven by the C++ client
-    tpr_amd/tpr_serve_bench (one client thread; the Python client's own call
-    overhead would otherwise bound the saturated rate).  Requests arrive as a
-    Poisson stream (mean args.rate per second); each brings a matrix of
-    N = 2^U{9..15} equations (diagonally dominant generator, generated on the
-    device and resident in HBM) and 1..64 right-hand sides d ~ U(-1,1) (ADI-
-    like reuse of the matrix); the server admits them against a 64 GB budget
-    and batches continuously (batching window 1024 right-hand sides / 100 us
-    while other batches run).  Reported: systems/s (right-hand sides solved
-    per second), p50/p99 latency from submit to completion; the same request
-    mix closed-loop with 256 outstanding (saturated throughput); the plain
-    per-system GPU solve (one launch per right-hand side, the reference's
-    CUDA model) on the first 300 requests; and the answers of a sample of
-    requests served through the Python binding against the oracle."""
+SERVE_WINDOW = {"min_batch_systems": 1024, "max_wait_us": 100.0}
+
+
+def serve_clients(args, seconds, closed_requests):
+    """The serving loop driven by the C++ client tpr_amd/tpr_serve_bench in
+    child processes (one client thread; the Python client's own call
+    overhead would otherwise bound the saturated rate): a Poisson stream of
+    args.rate requests per second for `seconds`, then the same request mix
+    closed-loop with 256 outstanding.  Each request brings a matrix of
+    N = 2^U{9..15} equations (diagonally dominant generator, generated on
+    the device and resident in HBM) and 1..64 right-hand sides d ~ U(-1,1);
+    64 GB budget; batching window 1024 right-hand sides / 100 us while other
+    batches run.  Run before this process touches the GPU."""
     import subprocess
-
-    import numpy as np
-
     exe = os.path.join(ROOT, "tpr_amd", "tpr_serve_bench")
     if not os.path.exists(exe):
         raise RuntimeError("tpr_amd/tpr_serve_bench is not built: run `python -m tpr_amd.build`")
-    # batching window: while batches run, the next one waits for 1024
-    # right-hand sides or 100 us (fewer, larger batches under load; no effect
-    # at the Poisson rate, where a request usually finds nothing in flight)
-    window = {"min_batch_systems": 1024, "max_wait_us": 100.0}
     common = ["--method", args.method, "--arith", args.arith, "--seed", "4044", "--budget-gb", "64",
-              "--min-batch", str(window["min_batch_systems"]), "--max-wait-us", str(window["max_wait_us"])]
+              "--min-batch", str(SERVE_WINDOW["min_batch_systems"]), "--max-wait-us",
+              str(SERVE_WINDOW["max_wait_us"])]
 
     def client(extra):
         p = subprocess.run([exe] + common + extra, capture_output=True, text=True, timeout=600)
@@ -667,24 +774,31 @@ def serve_workload(args):
             raise RuntimeError("tpr_serve_bench failed: %s" % p.stderr[-2000:])
         return json.loads(p.stdout.strip().splitlines()[-1])
 
-    # the clients run in child processes before this process touches the GPU
-    poisson = client(["--rate", str(args.rate), "--seconds", str(args.seconds)])
-    closed = client(["--rate", "0", "--requests", "20000", "--outstanding", "256"])
+    poisson = client(["--rate", str(args.rate), "--seconds", str(seconds)])
+    closed = client(["--rate", "0", "--requests", str(closed_requests), "--outstanding", "256"])
+    return poisson, closed
 
+
+def serve_request_mix(count):
+    import numpy as np
+    rng = np.random.default_rng(4044)
+    sizes = (1 << rng.integers(9, 16, count)).astype(np.int64)
+    nrhs = rng.integers(1, 65, count).astype(np.int64)
+    return sizes, nrhs
+
+
+def serve_per_system(args, count):
+    """The plain per-system GPU solve of the same kind of requests: one
+    tprh_solve_ex launch per right-hand side (the reference's CUDA model),
+    descriptors, workspaces and inputs prepared beforehand, only the
+    launches timed (wall clock, one final synchronisation)."""
     import ctypes
 
     import torch
 
     from tpr_amd import _lib
     from tpr_amd import batched as B
-    from tpr_amd.serve import Server
-
-    torch.cuda.set_device(0)
-    rng = np.random.default_rng(4044)
-    sizes = (1 << rng.integers(9, 16, 300)).astype(np.int64)
-    nrhs = rng.integers(1, 65, 300).astype(np.int64)
-    # plain per-system solves: one launch per right-hand side (descriptors,
-    # workspaces and inputs prepared beforehand; only the launches are timed)
+    sizes, nrhs = serve_request_mix(count)
     prepared = []
     for k in range(len(sizes)):
         n = int(sizes[k])
@@ -707,9 +821,56 @@ def serve_workload(args):
                 raise RuntimeError("tprh_solve_ex failed")
         tot_sys += r
     torch.cuda.synchronize()
-    base_wall = time.perf_counter() - t0
-    prepared = None
+    wall = time.perf_counter() - t0
+    return {"requests": int(len(sizes)), "systems": int(tot_sys), "systems_per_s": round(tot_sys / wall, 1)}
 
+
+def serve_accuracy(args, count):
+    """Every sixth request of the mix (up to two right-hand sides each)
+    served through the Python binding, against the oracle at s = min(N, 256)."""
+    import oracle as O
+    from tpr_amd.serve import Server
+    sizes, nrhs = serve_request_mix(count)
+    worst = 0.0
+    with Server(8 << 30, method=args.method, arith=args.arith) as srv:
+        reqs = []
+        for k in range(0, len(sizes), 6):
+            m = srv.matrix_generate(int(sizes[k]), seed=4044, sys=k)
+            reqs.append((k, m, srv.submit(m, int(min(nrhs[k], 2)), rhs_seed=4044)))
+        for k, m, r in reqs:
+            srv.wait(r)
+            x = srv.result(r).cpu().numpy()
+            n = int(sizes[k])
+            a, b, c, _, _ = O.gen_system(4044, k, n)
+            for j in range(x.shape[0]):
+                ref = O.solve_batch(args.method, a, b, c, O.gen_rhs(4044, k, j, n), n, min(n, 256))
+                worst = max(worst, O.rel_l2(x[j], ref))
+            srv.release(r)
+            srv.matrix_release(m)
+    return {"requests_checked": len(reqs), "max_rel_l2_vs_oracle": worst, "tolerance": 1e-5}
+
+
+def serve_summary(args, poisson, closed, base):
+    return {"workload": "Poisson arrivals at %.0f req/s for %.1f s (%d requests), N = 2^U{9..15}, 1..64 right-hand "
+                        "sides per matrix, 64 GB budget" % (args.rate, poisson["wall_s"], poisson["requests"]),
+            "systems_per_s": poisson["systems_per_s"], "latency_ms": poisson["latency_ms"],
+            "latency_from_arrival_ms": poisson.get("latency_from_arrival_ms"),
+            "saturated_systems_per_s": closed["systems_per_s"], "saturated_latency_ms": closed["latency_ms"],
+            "per_system_launches_systems_per_s": base["systems_per_s"],
+            "served_vs_per_system": round(closed["systems_per_s"] / base["systems_per_s"], 2),
+            "max_inflight_seen": poisson.get("max_inflight_seen")}
+
+
+def serve_workload(args):
+    """The serving loop under load (serve_clients): Poisson stream and the
+    saturated closed loop; beside them the plain per-system GPU solve (one
+    launch per right-hand side) on the first 300 requests of the mix; and
+    the answers of a sample of requests against the oracle."""
+    poisson, closed = serve_clients(args, args.seconds, 20000)
+
+    import torch
+    torch.cuda.set_device(0)
+    base = serve_per_system(args, 300)
     out = {
         "metric": METRIC, "value": poisson["systems_per_s"], "unit": "systems/s", "n_gpus": 1,
         "steps": poisson["requests"], "warmup": 7,
@@ -718,36 +879,19 @@ def serve_workload(args):
         "config": {"workload": "serving loop: Poisson arrivals at %.0f req/s for %.1f s (%d requests), N = 2^U{9..15}, "
                                "1..64 right-hand sides per matrix, matrices generated and resident in HBM, "
                                "64 GB budget, C++ client" % (args.rate, args.seconds, poisson["requests"]),
-                   "method": args.method, "arith": args.arith, "max_inflight": 4, "batching_window": window},
+                   "method": args.method, "arith": args.arith, "max_inflight": 4, "batching_window": SERVE_WINDOW},
         "latency_ms": poisson["latency_ms"],
+        "latency_from_arrival_ms": poisson.get("latency_from_arrival_ms"),
         # latency-bound at the offered load; the solve traffic (12 B per matrix
         # equation, 8 B per right-hand-side equation) against the HBM peak
         "roofline": {"bound": "hbm", "achieved": poisson["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(poisson["GBps"] / HBM_PEAK_GBS, 6), "traffic": None},
         "served": poisson,
         "saturated": closed,
-        "per_system_launches": {"requests": len(sizes), "systems": tot_sys,
-                                "systems_per_s": round(tot_sys / base_wall, 1)},
+        "per_system_launches": base,
     }
     if not args.no_cpu:
-        import oracle as O
-        worst = 0.0
-        with Server(8 << 30, method=args.method, arith=args.arith) as srv:
-            reqs = []
-            for k in range(0, len(sizes), 6):
-                m = srv.matrix_generate(int(sizes[k]), seed=4044, sys=k)
-                reqs.append((k, m, srv.submit(m, int(min(nrhs[k], 2)), rhs_seed=4044)))
-            for k, m, r in reqs:
-                srv.wait(r)
-                x = srv.result(r).cpu().numpy()
-                n = int(sizes[k])
-                a, b, c, _, _ = O.gen_system(4044, k, n)
-                for j in range(x.shape[0]):
-                    ref = O.solve_batch(args.method, a, b, c, O.gen_rhs(4044, k, j, n), n, min(n, 256))
-                    worst = max(worst, O.rel_l2(x[j], ref))
-                srv.release(r)
-                srv.matrix_release(m)
-        out["accuracy"] = {"requests_checked": len(reqs), "max_rel_l2_vs_oracle": worst, "tolerance": 1e-5}
+        out["accuracy"] = serve_accuracy(args, 300)
     print(json.dumps(out))
 
 
@@ -875,6 +1019,11 @@ def main():
     if args.workload == "config0":
         config0_workload(args)
         return
+    side = not args.no_side and args.shape is None and not args.eager
+    if side:
+        # the serving clients run in child processes before this one touches
+        # the GPU
+        poisson, closed = serve_clients(args, 2.0, 4000)
     g = gpu_leg(args)
     ms_per_step = g["event_ms"] / args.steps
     value = NSYS / (ms_per_step / 1e3)
@@ -932,6 +1081,15 @@ def main():
         base, acc = cpu_leg(args, g["host"], g["x0"])
         out["cpu_baseline"] = base
         out["accuracy"] = acc
+    if side:
+        # the other configurations of the metric, from short runs: the
+        # serving loop's latency (config 4), the uneven batch (config 2) and
+        # the precision variants (config 3)
+        sv = serve_summary(args, poisson, closed, serve_per_system(args, 100))
+        out["latency_ms"] = sv["latency_ms"]
+        out["serve"] = sv
+        out["uneven"] = uneven_side(args)
+        out["precision"] = precision_side(args)
     print(json.dumps(out))
 
 
