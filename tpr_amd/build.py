@@ -32,6 +32,8 @@ NSTAB_BIN = os.path.join(HERE, "tpr_nstab")
 SERVE_SRC = os.path.join(HERE, "cli", "tpr_serve_bench.cpp")
 SERVE_BIN = os.path.join(HERE, "tpr_serve_bench")
 VENDOR_SRC = os.path.join(HERE, "comparators", "rocsparse_gtsv.cpp")
+DIAG_SRC = os.path.join(HERE, "diag", "stage_clock.hip")
+DIAG_LIB = os.path.join(HERE, "libtpr_diag.so")
 VENDOR_LIB = os.path.join(HERE, "libtpr_rocsparse.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST_HIP = ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
@@ -94,6 +96,16 @@ def build_vendor():
     return VENDOR_LIB
 
 
+def build_diag():
+    """libtpr_diag.so: the per-stage clock build of the batched fast-PTPR
+    solve (tpr_amd/diag/stage_clock.hip, run by tpr_amd.stage_clock); a
+    diagnostic, not part of the solver library."""
+    tmp = DIAG_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", DIAG_SRC, "-o", tmp])
+    os.replace(tmp, DIAG_LIB)
+    return DIAG_LIB
+
+
 def build_nstab():
     tmp = NSTAB_BIN + ".tmp"
     _run(["g++", "-O2", "-std=c++17"] + HOST_HIP + ["-I" + os.path.join(ROOT, "include"), NSTAB_SRC, "-o", tmp,
@@ -118,7 +130,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     print("built", build_lib(args.j))
     if not args.lib:
-        for b in (build_cli(), build_vendor(), build_nstab(), build_serve_bench()):
+        for b in (build_cli(), build_vendor(), build_nstab(), build_serve_bench(), build_diag()):
             if b:
                 print("built", b)
 

@@ -1,0 +1,113 @@
+// stage_clock.hip -- a diagnostic build (libtpr_diag.so, not part of the
+// solver library): the batched fast-PTPR solve of one-wavefront systems on
+// 16-lane slices (the shape of the 4096 x N=1024, S=256 batch), assembled
+// from the same device functions as batched_kernel -- load_raw_t,
+// fast16_stage1, reduced_solve, fast16_stage3, store_e -- with the GPU's
+// constant 100 MHz clock (s_memrealtime) read between them.  Each wave
+// records six stamps: start, loads complete, stage 1 done, reduced system
+// done, stage 3 done, stores complete.  The waits that make "loads complete"
+// and "stores complete" exact (s_waitcnt), and the scheduling barriers and
+// empty uses of each stage's results that keep the stage between its stamps,
+// are the only differences from the shipped kernel; answers are the shipped kernel's bit for bit (checked by
+// tpr_amd/stage_clock.py).
+#include <hip/hip_runtime.h>
+
+#include "../csrc/slice_solvers.hpp"
+
+namespace tprh {
+
+constexpr int kClockBlock = 256;  // as batched_kernel for systems of one wavefront
+
+// Makes v's values inputs of an instruction that cannot move, so that the
+// stage producing them completes before the next stamp (the compiler would
+// otherwise sink arithmetic whose only use is later).
+__device__ __forceinline__ void keep(float v) { asm volatile("" ::"v"(v)); }
+template <int N>
+__device__ __forceinline__ void keep(const float (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i++) keep(v[i]);
+}
+
+__device__ __forceinline__ unsigned long long stamp() {
+    __builtin_amdgcn_sched_barrier(0);
+    const unsigned long long t = __builtin_amdgcn_s_memrealtime();
+    __builtin_amdgcn_sched_barrier(0);
+    return t;
+}
+
+template <int E>
+__global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const float* __restrict__ a,
+                                                                       const float* __restrict__ b,
+                                                                       const float* __restrict__ c,
+                                                                       const float* __restrict__ d,
+                                                                       float* __restrict__ x, int n, int m,
+                                                                       int nsys, int lg_lanes,
+                                                                       unsigned long long* __restrict__ stamps) {
+    extern __shared__ float lds[];
+    unsigned long long t[6];
+    t[0] = stamp();
+    const int sl = threadIdx.x >> lg_lanes;
+    const int tt = threadIdx.x & ((1 << lg_lanes) - 1);
+    float* my = lds + sl * lds_words_per_system(m);
+    const int sys = blockIdx.x * (blockDim.x >> lg_lanes) + sl;
+    const bool valid = sys < nsys;
+    const size_t x0 = valid ? (size_t)sys * (size_t)n : 0;
+    const size_t off = x0 + (size_t)tt * E;
+    RawPart<E> raw;
+    load_raw_t<E, float>(a, b, c, d, off, off, raw);
+    __builtin_amdgcn_s_waitcnt(0);
+    t[1] = stamp();
+    const int l = tt & 15;
+    const int j = tt >> 4;
+    Fast16State<E> st;
+    fast16_stage1<E>(raw, true, l, st);
+    keep(st.y);
+    keep(st.pc);
+    keep(st.q);
+    keep(st.eA), keep(st.eC), keep(st.eR), keep(st.sA), keep(st.sC), keep(st.sR);
+    t[2] = stamp();
+    const float* xr = reduced_solve<0, true>(my, m, j, l, 16, tt, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR);
+    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+    const float xe = xr[j];
+    const SliceEnd se = slice_end(my, m, j);
+    keep(xprev), keep(xe), keep(se.a), keep(se.c), keep(se.r);
+    t[3] = stamp();
+    float X[E];
+    fast16_stage3<E>(st, xprev, xe, se, l, X);
+    keep(X);
+    t[4] = stamp();
+    if (valid) store_e<E>(x + off, X);
+    __builtin_amdgcn_s_waitcnt(0);
+    t[5] = stamp();
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) stamps[(size_t)wave * 6 + k] = t[k];
+    }
+}
+
+}  // namespace tprh
+
+// The diagnostic solve: nsys systems of n = m * 16 * E equations with a
+// diagonal, E = s / 16 for s = 32 .. 256 and m * 16 <= 64 lanes per system;
+// stamps receives 6 values per wavefront (grid = ceil(nsys / (256 / (m*16)))
+// workgroups of 4 wavefronts).  Returns a hipError_t.
+extern "C" int tprh_diag_ptpr_stage_clock(const float* a, const float* b, const float* c, const float* d, float* x,
+                                          int n, int s, int nsys, unsigned long long* stamps, void* stream) {
+    using namespace tprh;
+    const int E = s / 16, m = n / s;
+    const int T = m * 16;
+    if (s < 32 || s > 256 || (s & (s - 1)) || n % s || T > 64 || (m & (m - 1))) return (int)hipErrorInvalidValue;
+    const int lg = __builtin_ctz((unsigned)T);
+    const int spb = kClockBlock / T;
+    const int grid = (nsys + spb - 1) / spb;
+    const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(m);
+    hipStream_t st = (hipStream_t)stream;
+    switch (E) {
+        case 2: hipLaunchKernelGGL(ptpr_stage_clock_kernel<2>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
+        case 4: hipLaunchKernelGGL(ptpr_stage_clock_kernel<4>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
+        case 8: hipLaunchKernelGGL(ptpr_stage_clock_kernel<8>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
+        default: hipLaunchKernelGGL(ptpr_stage_clock_kernel<16>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
+    }
+    return (int)hipGetLastError();
+}
