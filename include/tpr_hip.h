@@ -192,6 +192,20 @@ typedef struct tprh_solve_desc {
 size_t tprh_workspace_bytes(const tprh_solve_desc *desc);
 tprh_status tprh_solve_ex(const tprh_solve_desc *desc, void *stream);
 
+/* tprh_solve_ex without any allocation.  No counterpart in the reference:
+ * this is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * multi-workgroup path's workspace is a fixed device array of the library,
+ * TPRH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; desc's
+ * workspace must be NULL and workspace_bytes 0.  With chunk 0 the rounds are
+ * cut to fit that array (one system of the largest size, 2048 slices,
+ * needs 74 KB); an explicit chunk whose round needs more is refused with
+ * TPRH_ERROR_NOT_SUPPORTED before anything is launched.  Calls on one GPU
+ * share its array, so calls that overlap on the device race: order them on
+ * one stream.  Answers are those of tprh_solve_ex, bit for bit. */
+#define TPRH_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+tprh_status tprh_solve_ex_noalloc(const tprh_solve_desc *desc, void *stream);
+
 /* ---- batches of systems of different sizes ---------------------------------
  * One persistent launch for the whole batch.  n[k] (host array) is the size
  * of system k; the systems are stored back to back in this order in a, b, c,

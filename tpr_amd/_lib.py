@@ -68,6 +68,7 @@ class SolveDesc(ctypes.Structure):
 SIGNATURES.update({
     "tprh_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(SolveDesc)]),
     "tprh_solve_ex": (_i, [ctypes.POINTER(SolveDesc), _vp]),
+    "tprh_solve_ex_noalloc": (_i, [ctypes.POINTER(SolveDesc), _vp]),
     "tprh_uneven_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _vp, _i, _vp]),
     "tprh_uneven_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tprh_uneven_offset": (ctypes.c_longlong, [_vp, _i]),
