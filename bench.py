@@ -75,9 +75,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--shape", default=None, help="slice shape E,L (default: library default for S)")
     ap.add_argument("--chunk", type=int, default=0, help="precision workload: systems per round of the split path")
-    ap.add_argument("--split-form", type=int, default=0, choices=[0, 1, 2],
+    ap.add_argument("--split-form", type=int, default=0, choices=[0, 1, 2, 3],
                     help="precision workload: form of the multi-workgroup path (0 the library's choice, 1 two "
-                         "passes, 2 one pass)")
+                         "passes, 2 one pass, 3 pipelined)")
     ap.add_argument("--fp32-only", action="store_true",
                     help="precision workload: the fp32 solve alone (no fp16, refinement or per-system variants), "
                          "e.g. for counter runs")

@@ -183,8 +183,9 @@ typedef struct tprh_solve_desc {
                                (each slice stays in registers from load to store, 20 B/eq, where
                                every workgroup of a system can be resident at once, else two
                                passes; slower than two passes when a round has more workgroups
-                               than the device holds).  Exact answers are the same bits in
-                               either form. */
+                               than the device holds); 3 = pipelined (one persistent launch of
+                               rows, reduced-system and stage-3 tasks; slower than two passes
+                               on MI355X).  Exact answers are the same bits in every form. */
 } tprh_solve_desc;
 
 /* Workspace bytes tprh_solve_ex needs for desc (0 when none); (size_t)-1 if

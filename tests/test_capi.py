@@ -121,6 +121,13 @@ def test_workspace_sizing_without_device():
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
     d.coef_shared = 1                                             # one matrix: same workspace per system
     assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 128 * (3 + 1) + 256
+    for form in (1, 2, 3):                                        # every form takes the same workspace
+        d.split_form = form
+        assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 128 * (3 + 1) + 256
+    for form in (-1, 4):
+        d.split_form = form
+        assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
+    d.split_form = 0
     # a large system without a workspace is refused before any launch
     d.refine, d.coef, d.coef_shared = 0, 0, 0
     d.a = d.b = d.c = d.d = d.x = 16

@@ -266,17 +266,18 @@ def test_solve_ex_side_stream_keeps_its_workspace():
     assert bits_equal(x.cpu().numpy(), x_ref.cpu().numpy())
 
 
-@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("form", [1, 2, 3])
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
 @pytest.mark.parametrize("n,s,nsys,chunk", [SPLIT_CASES[0], SPLIT_CASES[2], SPLIT_CASES[3], SPLIT_CASES[6],
                                             (65536, 256, 70, 0)])
 def test_split_forms_bit_identical(form, method, n, s, nsys, chunk):
-    """Both forms of the multi-workgroup path, forced: two passes (stage 1
-    with last-arriver reduced solves, then stage 3) and one pass (slices in
-    registers throughout; 70 systems of 65536 are more blocks than the chip
-    holds at once, so its blocks come in several waves): exact answers
-    bit-identical to the oracle, fast ones within tolerance, with and
-    without a refinement step."""
+    """Every form of the multi-workgroup path, forced: two passes (stage 1
+    and the reduced systems, then stage 3), one pass (slices in registers
+    throughout; 70 systems of 65536 are more blocks than the chip holds at
+    once, so its blocks come in several waves) and pipelined (rows tasks and
+    stage-3 tasks in one launch, the system's last arriver solving its
+    reduced system): exact answers bit-identical to the oracle, fast ones
+    within tolerance, with and without a refinement step."""
     say("split form", form, method, n, s, nsys, chunk)
     a, b, c, d, xt = O.gen_batch(39, 0, nsys, n, O.GEN_WEAK)
     da, db, dc, dd = dev(a), dev(b), dev(c), dev(d)

@@ -106,8 +106,8 @@ def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sha
     storage (a, b, c as float16 tensors), one refinement step (refine=True:
     x += A^-1 (d - A x) with an FP64 residual), many right-hand sides of one
     matrix (shared=True: a, b, c of n values, d and x of nsys * n); split_form
-    0 / 1 / 2: the library's choice / two passes / one pass for systems above
-    one workgroup."""
+    0 / 1 / 2 / 3: the library's choice / two passes / one pass / pipelined
+    for systems above one workgroup."""
     import torch
     desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared, split_form)
     need = workspace_bytes(desc)

@@ -149,7 +149,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     if (d.coef != TPRH_STORE_F32 && d.coef != TPRH_STORE_F16) return fail(TPRH_ERROR_INVALID_VALUE, "bad storage");
     if (d.refine != 0 && d.refine != 1) return fail(TPRH_ERROR_INVALID_VALUE, "refine must be 0 or 1");
     if (d.coef_shared != 0 && d.coef_shared != 1) return fail(TPRH_ERROR_INVALID_VALUE, "coef_shared must be 0 or 1");
-    if (d.split_form < 0 || d.split_form > 2) return fail(TPRH_ERROR_INVALID_VALUE, "split_form must be 0, 1 or 2");
+    if (d.split_form < 0 || d.split_form > 3) return fail(TPRH_ERROR_INVALID_VALUE, "split_form must be 0 .. 3");
     if (d.refine && d.coef != TPRH_STORE_F32)
         return fail(TPRH_ERROR_INVALID_VALUE, "refinement needs fp32 coefficients for the residual");
     const bool explicit_shape = d.E != 0 || d.L != 0;

@@ -41,7 +41,7 @@ hipError_t launch_el(const ServeSplitLaunch& p, hipStream_t stream) {
     int rthreads = m < 1024 ? m : 1024;
     rthreads = (rthreads + 63) / 64 * 64;
     const size_t rshmem = sizeof(float) * 10 * (size_t)m;
-    hipLaunchKernelGGL((rows_kernel<E, L, MODE, true, float, false, TableLoc>), dim3(grid), dim3(threads), 0, stream,
+    hipLaunchKernelGGL((rows_kernel<METHOD, E, L, MODE, true, float, false, TableLoc>), dim3(grid), dim3(threads), 0, stream,
                        loc, lg_m, nslices);
     hipLaunchKernelGGL((reduced_kernel<METHOD, TableLoc>), dim3(p.nsys), dim3(rthreads), rshmem, stream, loc);
     hipLaunchKernelGGL((stage3_kernel<METHOD, E, L, MODE, true, float, false, TableLoc>), dim3(grid), dim3(threads), 0,

@@ -2,7 +2,7 @@
 //
 // The reference needs every block of a system co-resident for its grid-wide
 // barriers and limits n to s^2 (src/ptpr.cu:543,581,655).  Here a batch is
-// solved in rounds of `chunk` systems, each round in one of two forms:
+// solved in rounds of `chunk` systems, each round in one of three forms:
 //
 // Two passes (split_kernels.hpp), the library's form:
 //   K1 rows_kernel   : every lane group reduces its slices and writes two
@@ -19,20 +19,31 @@
 // PCR the final value of a slice's first (last) row depends only on the rows
 // at offsets 0 mod 2^p (-1 mod 2^p) of the previous levels, updated with the
 // same formulas in the same order as TPR's tree stage 1 (src/tpr.cpp:142-204
-// vs src/ptpr.cpp:140-182), so K1 runs the O(S) tree for both methods and its
-// rows are bit-identical to the full PCR's.
+// vs src/ptpr.cpp:140-182), so in exact arithmetic K1 runs the O(S) tree for
+// both methods and its rows are bit-identical to the full PCR's.  In fast
+// arithmetic, PTPR on 16-lane slices runs the two-level stage 1 in K1 and K3
+// (as the one-workgroup path does), which also keeps both kernels at 72-78
+// VGPRs, 6-7 waves per SIMD: on MI355X the 256 x 65536 batch takes 41 + 6 +
+// 50 us this way against 42 + 6 + 58 us with the register PCR in K3.
 //
 // One pass (persistent_split_kernel), on request (split_form 2): each block
 // keeps its slices in registers from the load to the store and waits for
 // its system's reduced solve (20 B/eq).  It needs every block of a system
 // resident at once, and each wave of resident blocks exposes a reduced
-// solve's latency with nothing to overlap it: on MI355X config 3 (256 x
-// 65536, four waves of blocks) takes 0.25 ms this way against 0.11 ms in two
-// passes, so the library does not choose it by itself.
+// solve's latency with nothing to overlap it: 0.25 ms for 256 x 65536.
 //
-// Answers are the same bits in either form in exact arithmetic (the same
-// operations in the same order); in fast arithmetic the one-pass form runs
-// PTPR's two-level stage 1 on 16-lane slices, as the one-workgroup path does.
+// Pipelined (pipelined_split_kernel), on request (split_form 3): one
+// persistent launch of rows, reduced and stage-3 tasks in ticket order, the
+// reduced solves overlapping other systems' loads and the reloads meant to
+// hit the last-level cache.  0.136-0.149 ms for 256 x 65536: its waits on
+// device-coherent atomics (microseconds each behind the memory traffic) and
+// its 108 VGPRs (4 waves per SIMD, so fewer bytes in flight) cost more than
+// the reloads it saves, which the last-level cache serves only about 1.4x
+// faster than HBM on this part.
+//
+// The library chooses two passes.  Exact answers are the same bits in every
+// form (the same operations in the same order); fast answers agree within
+// rounding (the forms differ in where stage 1's end rows come from).
 #include <hip/hip_runtime.h>
 
 #include "split_kernels.hpp"
@@ -73,7 +84,7 @@ hipError_t launch_round_two_pass(const SplitArgs& p, size_t base, int cs, int m,
     const int lg_m = __builtin_ctz((unsigned)m);
     const long long nslices = (long long)cs * m;
     const int grid = (int)((nslices * L + 255) / 256);
-    hipLaunchKernelGGL((rows_kernel<E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid), dim3(256), 0, stream,
+    hipLaunchKernelGGL((rows_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid), dim3(256), 0, stream,
                        loc, lg_m, nslices);
     // K2 touches only the rows and x_ed: one instantiation for every storage type
     BatchLoc<float> rl{};
@@ -100,6 +111,12 @@ int persistent_capacity(K kernel, size_t lds) {
     return cus * occ;
 }
 
+// Lag of the pipelined form in rows tasks: twice the workgroups resident at
+// once (capacity; its query failing gives 0), so that a task's waits are
+// normally over when it starts (measured on MI355X at 0.5, 1, 2 and 4 times
+// the capacity: 0.149, 0.140, 0.139, 0.136 ms for 256 x 65536).
+int pipeline_lag(int capacity) { return 2 * capacity; }
+
 template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
 hipError_t launch_round_persistent(const SplitArgs& p, size_t base, int cs, int m, int lg_bps, int capacity,
                                    size_t lds, hipStream_t stream) {
@@ -110,6 +127,22 @@ hipError_t launch_round_persistent(const SplitArgs& p, size_t base, int cs, int 
     const int grid = (int)(blocks < capacity ? blocks : capacity);
     hipLaunchKernelGGL((persistent_split_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid),
                        dim3(kPersistThreads), lds, stream, loc, p.sync, cs, lg_bps);
+    return hipGetLastError();
+}
+
+// Pipelined round: one persistent launch (at most `capacity` workgroups)
+// running a rows task and a stage-3 task per block of slices, taken in
+// ticket order with a lag of `lag` blocks between the two.
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE>
+hipError_t launch_round_pipelined(const SplitArgs& p, size_t base, int cs, int m, int lg_bps, int capacity, int lag,
+                                  size_t lds, hipStream_t stream) {
+    const BatchLoc<CT> loc = round_loc<CT>(p, base, m);
+    const int words = kSplitSyncStride * (cs + 1);
+    hipLaunchKernelGGL(clear_words_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, p.sync, words);
+    const long long tasks = pipeline_tickets(cs, lg_bps);
+    const int grid = (int)(tasks < capacity ? tasks : (capacity > 0 ? capacity : 1));
+    hipLaunchKernelGGL((pipelined_split_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>), dim3(grid),
+                       dim3(kPersistThreads), lds, stream, loc, p.sync, cs, lg_bps, lag, NoStageClock{});
     return hipGetLastError();
 }
 
@@ -133,12 +166,27 @@ hipError_t launch_split_variant(const SplitArgs& p, int m, hipStream_t stream) {
     // and only where every block of a system can be resident at once
     const int capacity = p.form == 2 ? persistent_capacity(pk, lds) : 0;
     const bool one_pass = p.form == 2 && capacity >= bps;
+    int lag = 0, qcap = 0;
+    if (p.form == 3) {
+        const auto qk = pipelined_split_kernel<METHOD, E, L, MODE, HAS_B, CT, REFINE, BatchLoc<CT>>;
+        static std::atomic<unsigned long long> qlds_set{0};
+        if ((e = ensure_lds_limit(reinterpret_cast<const void*>(qk), lds_max, qlds_set)) != hipSuccess) return e;
+        qcap = persistent_capacity(qk, lds);
+        // the lag: about the workgroups the device holds at once, so that a
+        // stage-3 task is normally taken after its system's reduced solve
+        lag = pipeline_lag(qcap);
+    }
     for (int s0 = 0; s0 < p.nsys; s0 += p.chunk) {
         const int cs = (p.nsys - s0 < p.chunk) ? p.nsys - s0 : p.chunk;
         const size_t base = (size_t)s0 * p.n;
-        e = one_pass ? launch_round_persistent<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lg_bps, capacity,
-                                                                                     lds, stream)
-                     : launch_round_two_pass<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lds, stream);
+        if (p.form == 3)
+            e = launch_round_pipelined<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lg_bps, qcap, lag, lds,
+                                                                              stream);
+        else if (one_pass)
+            e = launch_round_persistent<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lg_bps, capacity, lds,
+                                                                               stream);
+        else
+            e = launch_round_two_pass<METHOD, E, L, MODE, HAS_B, CT, REFINE>(p, base, cs, m, lds, stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

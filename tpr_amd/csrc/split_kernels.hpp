@@ -48,7 +48,13 @@ __device__ __forceinline__ void slice_index(long long gs, int lg_m, int l, long 
     lo = (size_t)j * (E * L) + (size_t)l * E;
 }
 
-template <int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
+// Stage 1 in fast arithmetic for PTPR on 16-lane slices takes the two-level
+// form (a Thomas sweep in each lane, PCR across the lanes' last rows), as the
+// one-workgroup path does; everything else the reference's order.
+template <int METHOD, int E, int L, int MODE>
+constexpr bool kSplitTwoLevel = METHOD == 0 && MODE == kFast && L == 16 && E >= 2;
+
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
 __global__ __launch_bounds__(256) void rows_kernel(Loc loc, int lg_m, long long nslices) {
     const int l = threadIdx.x % L;
     const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;  // slice over the round
@@ -67,29 +73,39 @@ __global__ __launch_bounds__(256) void rows_kernel(Loc loc, int lg_m, long long 
         if (valid && l == 0) row[6] = xo[0];
         if (valid && l == L - 1) row[7] = xo[E - 1];
     }
-    float A[E], C[E], R[E];
-    normalise<E, MODE>(raw, HAS_B, A, C, R);
-    tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+    float st_a, st_c, st_r, ed_a, ed_c, ed_r;
+    if constexpr (kSplitTwoLevel<METHOD, E, L, MODE>) {
+        Fast16State<E> fs;
+        fast16_stage1<E>(raw, HAS_B, l, fs);
+        st_a = fs.sA, st_c = fs.sC, st_r = fs.sR;
+        ed_a = fs.eA, ed_c = fs.eC, ed_r = fs.eR;
+    } else {
+        float A[E], C[E], R[E];
+        normalise<E, MODE>(raw, HAS_B, A, C, R);
+        tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+        st_a = A[0], st_c = C[0], st_r = R[0];
+        ed_a = A[E - 1], ed_c = C[E - 1], ed_r = R[E - 1];
+    }
     if (valid) {
         if (l == 0) {
-            row[0] = A[0];
-            row[1] = C[0];
-            row[2] = R[0];
+            row[0] = st_a;
+            row[1] = st_c;
+            row[2] = st_r;
         }
         if (l == L - 1) {
-            row[3] = A[E - 1];
-            row[4] = C[E - 1];
-            row[5] = R[E - 1];
+            row[3] = ed_a;
+            row[4] = ed_c;
+            row[5] = ed_r;
         }
     }
 }
 
-// The reduced system of one system from its m slices' boundary rows
-// (sysrows, kRowWords floats per slice): E_st += s1 E_ed
+// reduced_rows_lds: the reduced system of one system from its m slices'
+// boundary rows (sysrows, kRowWords floats per slice): E_st += s1 E_ed
 // (src/ptpr.cpp:193-211, src/tpr.cpp:327-340), the reduced rows
 // (src/ptpr.cpp:218-241) and their solve in LDS, PCR for PTPR
 // (src/pcr.cpp:15-89) or CR for TPR (src/cr.cpp:23-116), by the nthr threads
-// t = 0 .. nthr-1 of one workgroup; x at every slice end goes to xed.
+// t = 0 .. nthr-1 of one workgroup; returns x at every slice end, in lds.
 // lds: 10 m floats.  Every thread of the workgroup must call it.
 // Loads and stores of the data blocks exchange inside one launch (boundary
 // rows, x at slice ends, counters, flags): atomic at device scope, so that
@@ -104,8 +120,7 @@ __device__ __forceinline__ void st_dev(float* p, float v) {
 }
 
 template <int METHOD, bool COHERENT = false>
-__device__ __forceinline__ void reduced_rows_solve(const float* sysrows, float* xed, int m, float* lds, int t,
-                                                   int nthr) {
+__device__ __forceinline__ const float* reduced_rows_lds(const float* sysrows, int m, float* lds, int t, int nthr) {
     auto ld = [](const float* p) { return COHERENT ? ld_dev(p) : *p; };
     float* st_a = lds;
     float* st_c = st_a + m;
@@ -117,19 +132,23 @@ __device__ __forceinline__ void reduced_rows_solve(const float* sysrows, float* 
     float* rc1 = ra1 + m;
     float* rr1 = rc1 + m;
     float* xo = rr1 + m;
+    // E_ed waits in ra1 / rc1 / rr1 (free until the solve) between the passes
     for (int j = t; j < m; j += nthr) {
         const float* r = sysrows + (size_t)j * kRowWords;
-        const float c_ed = ld(r + 4);
-        const float s1 = (fabsf(c_ed) < FLT_EPSILON) ? -1.0f : -ld(r + 1) / c_ed;
-        st_a[j] = ld(r + 0) + s1 * ld(r + 3);
+        const float a_st = ld(r + 0), c_st = ld(r + 1), r_st = ld(r + 2);
+        const float a_ed = ld(r + 3), c_ed = ld(r + 4), r_ed = ld(r + 5);
+        const float s1 = (fabsf(c_ed) < FLT_EPSILON) ? -1.0f : -c_st / c_ed;
+        st_a[j] = a_st + s1 * a_ed;
         st_c[j] = s1;
-        st_r[j] = ld(r + 2) + s1 * ld(r + 5);
+        st_r[j] = r_st + s1 * r_ed;
+        ra1[j] = a_ed;
+        rc1[j] = c_ed;
+        rr1[j] = r_ed;
     }
     __syncthreads();
     for (int j = t; j < m; j += nthr) {
         // reduced row j from (E_ed(j), E_st(j+1))
-        const float* r = sysrows + (size_t)j * kRowWords;
-        const float ak = ld(r + 3), ck = ld(r + 4), rk = ld(r + 5);
+        const float ak = ra1[j], ck = rc1[j], rk = rr1[j];
         if (j < m - 1) {
             const float akr = st_a[j + 1], ckr = st_c[j + 1], rkr = st_r[j + 1];
             const float inv = 1.0f / (1.0f - akr * ck);
@@ -143,13 +162,19 @@ __device__ __forceinline__ void reduced_rows_solve(const float* sysrows, float* 
         }
     }
     __syncthreads();
-    const float* res;
     if constexpr (METHOD == 0) {
-        res = lds_pcr(ra0, rc0, rr0, ra1, rc1, rr1, m, t, nthr);
+        return lds_pcr(ra0, rc0, rr0, ra1, rc1, rr1, m, t, nthr);
     } else {
         lds_cr(ra0, rc0, rr0, xo, m, t, nthr);
-        res = xo;
+        return xo;
     }
+}
+
+// As reduced_rows_lds, x at every slice end then written to xed.
+template <int METHOD, bool COHERENT = false>
+__device__ __forceinline__ void reduced_rows_solve(const float* sysrows, float* xed, int m, float* lds, int t,
+                                                   int nthr) {
+    const float* res = reduced_rows_lds<METHOD, COHERENT>(sysrows, m, lds, t, nthr);
     for (int j = t; j < m; j += nthr) {
         if constexpr (COHERENT) st_dev(xed + j, res[j]);
         else xed[j] = res[j];
@@ -191,16 +216,28 @@ __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long lon
         else xr = (j < m - 1) ? rows[(size_t)(j + 1) * kRowWords + 6] : 0.0f;
         residual_fp64<E, HAS_B>(xo, xl, xr, raw);
     }
-    float A[E], C[E], R[E], X[E];
-    normalise<E, MODE>(raw, HAS_B, A, C, R);
+    float X[E];
     const float* xed = loc.Xed(q);
     const float xe = xed[j];
     const float xprev = (j > 0) ? xed[j - 1] : 0.0f;
-    if constexpr (METHOD == 0) {
+    if constexpr (kSplitTwoLevel<METHOD, E, L, MODE>) {
+        Fast16State<E> fs;
+        fast16_stage1<E>(raw, HAS_B, l, fs);
+        const int src = (int)(threadIdx.x & ~(unsigned)(L - 1)) + (L - 1);  // the slice's last lane
+        SliceEnd se;
+        se.a = __shfl(fs.eA, src, 64);
+        se.c = __shfl(fs.eC, src, 64);
+        se.r = __shfl(fs.eR, src, 64);
+        fast16_stage3<E>(fs, xprev, xe, se, l, X);
+    } else if constexpr (METHOD == 0) {
+        float A[E], C[E], R[E];
+        normalise<E, MODE>(raw, HAS_B, A, C, R);
         ptpr_stage1<E, L, MODE>(A, C, R, l);
         const float* row = rows + (size_t)j * kRowWords;
         ptpr_stage3<E, L, MODE>(A, C, R, xprev, xe, SliceEnd{row[3], row[4], row[5]}, l, X);
     } else {
+        float A[E], C[E], R[E];
+        normalise<E, MODE>(raw, HAS_B, A, C, R);
         float Ao[E], Co[E], Ro[E];
 #pragma unroll
         for (int i = 0; i < E; i++) {
@@ -379,4 +416,274 @@ __global__ __launch_bounds__(kPersistThreads) void persistent_split_kernel(Loc l
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined form: one persistent launch per round whose workgroups take
+// tasks in ticket order from a counter (sync[0]).  Three kinds of task:
+//   rows task    (block i of kPersistThreads / L slices, block i % bps of
+//                system i / bps): load the slices, stage 1, write their
+//                boundary rows (E_st before its update, E_ed; with
+//                refinement also x at the slice edges), count the arrival in
+//                the system's counter;
+//   reduced task (system q): wait until all bps rows tasks of q have
+//                arrived, solve the reduced system (reduced_rows_solve),
+//                raise q's flag;
+//   stage-3 task (block i): load the slices again, wait for the flag, redo
+//                stage 1, run stage 3 and store x.
+// Two streams of tasks share the tickets (pipeline_task, launch_split.hpp):
+// the rows tasks, and a lag D behind them each system's reduced task
+// followed by the stage-3 tasks of the system g before it.  So a system's reduced
+// task comes about 2 D tickets after its rows tasks and its stage-3 tasks
+// about 2 g (bps + 1) tickets after that: with D about the number of
+// workgroups resident at once and g = D / (2 bps), each wait is normally
+// over when a task starts, and a stage-3 task's reload finds its slices in
+// the device's last-level cache, so HBM sees the inputs about once.
+// Device-coherent operations wait behind the memory traffic (microseconds at
+// full bandwidth), so no workgroup waits for one where it need not: the next
+// ticket is requested as a task starts and read at its end; a stage-3 task
+// reads its flag ahead of its loads; a rows task's row stores complete, and
+// its arrival is counted, while the workgroup's next task loads (before
+// anything in that task waits for another workgroup).
+// No wait can block for ever, whatever the residency: every task depends
+// only on tasks of lower tickets (rows tasks of q < reduced task of q <
+// stage-3 tasks of q, for any D >= 2 bps), a workgroup takes tickets in
+// increasing order, holds at most its current and its next one, and counts
+// its previous rows task's arrival before it waits for anything, so the
+// lowest unfinished ticket is some workgroup's current task and all it
+// waits for is done or under way.  The waits are also bounded like the
+// one-pass form's, with timeouts counted in sync[1].
+// Answers are the same bits as the one-pass form's: the same stage 1 (the
+// two-level form for fast PTPR on 16-lane slices; in exact arithmetic the
+// O(S) tree for the rows, whose end rows equal the in-slice PCR's bit for
+// bit), the same reduced solve and the same stage 3.
+__device__ __forceinline__ bool wait_count(int* counter, int target) {
+    for (long long i = 0; i < kPersistSpinLimit; i++) {
+        if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        __builtin_amdgcn_s_sleep(8);
+    }
+    return false;
+}
+
+// Points of a pipelined task at which the diagnostic build reads the GPU
+// clock (libtpr_diag.so passes a clock that does; this one does nothing and
+// compiles away): mark(0) task start, mark(1) loads complete / arrivals
+// seen, mark(2) stage 1 done / reduced system solved, mark(3) rows stored /
+// flag seen / flag raised, mark(4) x_ed read, mark(5) next ticket read;
+// loads_done() makes mark(1) exact, keep() keeps a stage before its mark,
+// task() records the ticket's stamps.
+struct NoStageClock {
+    __device__ __forceinline__ void mark(int) {}
+    __device__ __forceinline__ void loads_done() {}
+    __device__ __forceinline__ void keep(float) {}
+    __device__ __forceinline__ void task(int, int) {}
+};
+
+template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc,
+          typename Clock = NoStageClock>
+__global__ __launch_bounds__(kPersistThreads) void pipelined_split_kernel(Loc loc, int* __restrict__ sync, int nsys,
+                                                                          int lg_bps, int lag, Clock clk) {
+    extern __shared__ float lds[];  // 10 m floats: the reduced solve
+    __shared__ int s_ticket;
+    constexpr int kSlices = kPersistThreads / L;
+    constexpr bool kTwoLevel = METHOD == 0 && MODE == kFast && L == 16 && E >= 2;
+    const int bps = 1 << lg_bps;
+    const int D = pipeline_d(lag, nsys << lg_bps, bps);
+    const int g = pipeline_gap(D, bps, nsys);
+    const int ntk = pipeline_tickets(nsys, lg_bps);
+    const int l = threadIdx.x % L;
+    const int jb = threadIdx.x / L;  // slice inside the block
+    int pend_q = -1;                 // system of the previous rows task, its arrival not yet counted (uniform)
+    // counts the previous rows task's arrival once every wave's row stores
+    // are complete (device-coherent, visible to the whole chip)
+    auto arrive = [&]() {
+        if (pend_q >= 0) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_fetch_add(sync + (size_t)kSplitSyncStride * (pend_q + 1), 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            pend_q = -1;
+        }
+    };
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&sync[0], 1);
+    __syncthreads();
+    int tk = s_ticket;
+    while (tk < ntk) {  // uniform
+        __syncthreads();  // every thread has read s_ticket; lds is free again
+        clk.mark(0);
+        int nxt = ntk;
+        if (threadIdx.x == 0) nxt = atomicAdd(&sync[0], 1);
+        const PipeTask task = pipeline_task(tk, nsys, lg_bps, D, g);
+        const int kind = task.kind, i = task.i;
+        const int q = i >> lg_bps;
+        int* sys_sync = sync + (size_t)kSplitSyncStride * (q + 1);
+        int* flag = sys_sync + kSplitSyncStride / 2;
+        if (kind == kPipeReduced) {
+            arrive();
+            if (threadIdx.x == 0 && !wait_count(sys_sync, bps))
+                __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            clk.mark(1);
+            reduced_rows_solve<METHOD, true>(loc.Rows(q), loc.Xed(q), loc.m, lds, threadIdx.x, kPersistThreads);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            clk.mark(2);
+            if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            clk.mark(3);
+        } else {
+            const int j = (i & (bps - 1)) * kSlices + jb;  // slice inside the system
+            const size_t lo = (size_t)j * (E * L) + (size_t)l * E;
+            float* rows = loc.Rows(q);
+            float* row = rows + (size_t)j * kRowWords;
+            float* x = loc.X(q);
+            int seen = 0;  // the flag, read ahead of the loads (thread 0)
+            if (kind == kPipeStage3 && threadIdx.x == 0)
+                seen = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            RawPart<E> raw;
+            load_raw_t<E, CT>(loc.A(q), HAS_B ? loc.B(q) : nullptr, loc.C(q), loc.D(q), lo, lo, raw);
+            float xo[E];
+            if (kind == kPipeStage3) {
+                if constexpr (REFINE) load_e<E>(x + lo, xo);
+            }
+            arrive();
+            clk.loads_done();
+            clk.mark(1);
+            if constexpr (kTwoLevel && !REFINE) {
+                // stage 1 needs nothing from other workgroups: one body for
+                // both kinds, run before a stage-3 task waits for its flag
+                Fast16State<E> fs;
+                fast16_stage1<E>(raw, HAS_B, l, fs);
+                clk.keep(fs.eA), clk.keep(fs.eC), clk.keep(fs.eR), clk.keep(fs.sA), clk.keep(fs.sC), clk.keep(fs.sR);
+                clk.mark(2);
+                if (kind == kPipeRows) {
+                    if (l == 0) {
+                        st_dev(row + 0, fs.sA);
+                        st_dev(row + 1, fs.sC);
+                        st_dev(row + 2, fs.sR);
+                    }
+                    if (l == L - 1) {
+                        st_dev(row + 3, fs.eA);
+                        st_dev(row + 4, fs.eC);
+                        st_dev(row + 5, fs.eR);
+                    }
+                    pend_q = q;
+                    clk.mark(3);
+                } else {
+                    if (threadIdx.x == 0 && !seen && !wait_flag(flag))
+                        __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __syncthreads();
+                    clk.mark(3);
+                    const float* xed = loc.Xed(q);
+                    const float xe = ld_dev(xed + j);
+                    const float xprev = (j > 0) ? ld_dev(xed + j - 1) : 0.0f;
+                    clk.keep(xe), clk.keep(xprev);
+                    clk.mark(4);
+                    const int src = (int)(threadIdx.x & ~(unsigned)(L - 1)) + (L - 1);  // the slice's last lane
+                    SliceEnd se;
+                    se.a = __shfl(fs.eA, src, 64);
+                    se.c = __shfl(fs.eC, src, 64);
+                    se.r = __shfl(fs.eR, src, 64);
+                    float X[E];
+                    fast16_stage3<E>(fs, xprev, xe, se, l, X);
+                    store_e<E>(x + lo, X);
+                }
+            } else if (kind == kPipeRows) {
+                if constexpr (REFINE) {
+                    // no x of this system is stored before its flag, which
+                    // comes after every rows task of the system has loaded x
+                    float xl, xr;
+                    load_x_with_halo<E>(x, 0, lo, loc.n, xo, xl, xr);
+                    residual_fp64<E, HAS_B>(xo, xl, xr, raw);
+                    if (l == 0) st_dev(row + 6, xo[0]);
+                    if (l == L - 1) st_dev(row + 7, xo[E - 1]);
+                }
+                float st_a, st_c, st_r, ed_a, ed_c, ed_r;
+                if constexpr (kTwoLevel) {
+                    Fast16State<E> fs;
+                    fast16_stage1<E>(raw, HAS_B, l, fs);
+                    st_a = fs.sA, st_c = fs.sC, st_r = fs.sR;
+                    ed_a = fs.eA, ed_c = fs.eC, ed_r = fs.eR;
+                } else {
+                    float A[E], C[E], R[E];
+                    normalise<E, MODE>(raw, HAS_B, A, C, R);
+                    tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+                    st_a = A[0], st_c = C[0], st_r = R[0];
+                    ed_a = A[E - 1], ed_c = C[E - 1], ed_r = R[E - 1];
+                }
+                if (l == 0) {
+                    st_dev(row + 0, st_a);
+                    st_dev(row + 1, st_c);
+                    st_dev(row + 2, st_r);
+                }
+                if (l == L - 1) {
+                    st_dev(row + 3, ed_a);
+                    st_dev(row + 4, ed_c);
+                    st_dev(row + 5, ed_r);
+                }
+                pend_q = q;
+            } else {
+                if (threadIdx.x == 0 && !seen && !wait_flag(flag))
+                    __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                if constexpr (REFINE) {
+                    // inside the slice the neighbours belong to this wave,
+                    // whose loads precede its stores; across slice edges, the
+                    // values the rows tasks saved before any x was stored
+                    const int m = loc.m;
+                    float xl, xr;
+                    if (l > 0) xl = x[lo - 1];
+                    else xl = (j > 0) ? ld_dev(rows + (size_t)(j - 1) * kRowWords + 7) : 0.0f;
+                    if (l < L - 1) xr = x[lo + E];
+                    else xr = (j < m - 1) ? ld_dev(rows + (size_t)(j + 1) * kRowWords + 6) : 0.0f;
+                    residual_fp64<E, HAS_B>(xo, xl, xr, raw);
+                }
+                const float* xed = loc.Xed(q);
+                const float xe = ld_dev(xed + j);
+                const float xprev = (j > 0) ? ld_dev(xed + j - 1) : 0.0f;
+                const int src = (int)(threadIdx.x & ~(unsigned)(L - 1)) + (L - 1);  // the slice's last lane
+                float X[E];
+                if constexpr (kTwoLevel) {
+                    Fast16State<E> fs;
+                    fast16_stage1<E>(raw, HAS_B, l, fs);
+                    SliceEnd se;
+                    se.a = __shfl(fs.eA, src, 64);
+                    se.c = __shfl(fs.eC, src, 64);
+                    se.r = __shfl(fs.eR, src, 64);
+                    fast16_stage3<E>(fs, xprev, xe, se, l, X);
+                } else {
+                    float A[E], C[E], R[E];
+                    normalise<E, MODE>(raw, HAS_B, A, C, R);
+                    if constexpr (METHOD == 0) {
+                        ptpr_stage1<E, L, MODE>(A, C, R, l);
+                        SliceEnd se;
+                        se.a = __shfl(A[E - 1], src, 64);
+                        se.c = __shfl(C[E - 1], src, 64);
+                        se.r = __shfl(R[E - 1], src, 64);
+                        ptpr_stage3<E, L, MODE>(A, C, R, xprev, xe, se, l, X);
+                    } else {
+                        float Ao[E], Co[E], Ro[E];
+#pragma unroll
+                        for (int c = 0; c < E; c++) {
+                            Ao[c] = A[c];
+                            Co[c] = C[c];
+                            Ro[c] = R[c];
+                        }
+                        tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
+                        tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xe, l, X);
+                    }
+                }
+                if constexpr (REFINE) {
+#pragma unroll
+                    for (int c = 0; c < E; c++) X[c] = xo[c] + X[c];
+                }
+                store_e<E>(x + lo, X);
+            }
+        }
+        if (threadIdx.x == 0) s_ticket = nxt;
+        __syncthreads();
+        clk.mark(5);
+        clk.task(tk, kind);
+        tk = s_ticket;
+    }
+    arrive();
+}
 }  // namespace tprh

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../csrc/slice_solvers.hpp"
+#include "../csrc/split_kernels.hpp"
 
 namespace tprh {
 
@@ -86,6 +87,36 @@ __global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const flo
     }
 }
 
+
+// The multi-workgroup path's pipelined form: the shipped
+// pipelined_split_kernel itself, instantiated for fast PTPR on 16 x 16
+// slices with a diagonal, fp32, without refinement, with a clock that reads
+// the GPU's 100 MHz counter at the kernel's marks (see NoStageClock in
+// split_kernels.hpp) and waits for the loads before mark(1).  Per ticket,
+// 8 words: the six stamps (a mark a task does not pass keeps the value of
+// the last one it passed) and kind (0 rows, 1 reduced, 2 stage 3) | workgroup << 8.
+struct SplitStampClock {
+    unsigned long long* out;
+    unsigned long long t[6];
+    __device__ __forceinline__ void mark(int k) {
+        const unsigned long long v = stamp();
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+            if (c >= k) t[c] = v;
+    }
+    __device__ __forceinline__ void loads_done() { __builtin_amdgcn_s_waitcnt(0); }
+    __device__ __forceinline__ void keep(float v) { asm volatile("" ::"v"(v)); }
+    __device__ __forceinline__ void task(int tk, int kind) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) out[(size_t)tk * 8 + c] = t[c];
+            out[(size_t)tk * 8 + 6] = (unsigned long long)kind | ((unsigned long long)blockIdx.x << 8);
+        }
+    }
+};
+
+constexpr auto split_clock_kernel =
+    pipelined_split_kernel<0, 16, 16, kFast, true, float, false, BatchLoc<float>, SplitStampClock>;
 }  // namespace tprh
 
 // The diagnostic solve: nsys systems of n = m * 16 * E equations with a
@@ -109,5 +140,54 @@ extern "C" int tprh_diag_ptpr_stage_clock(const float* a, const float* b, const 
         case 8: hipLaunchKernelGGL(ptpr_stage_clock_kernel<8>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
         default: hipLaunchKernelGGL(ptpr_stage_clock_kernel<16>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
     }
+    return (int)hipGetLastError();
+}
+
+// The pipelined split solve of nsys fast-PTPR systems of n equations, s = 256
+// (16 x 16 slices, m = n / 256 slices per system), with a diagonal;
+// workspace as tprh_workspace_bytes gives for chunk = nsys; lag in rows tasks
+// (0: the library's, twice the workgroups the device holds at once); the
+// launch is persistent (the workgroups the device holds at once).  stamps
+// receives 8 words per ticket, tprh_diag_split_tickets(n, nsys) of them.
+// Returns a hipError_t.
+extern "C" long long tprh_diag_split_tickets(int n, int nsys) {
+    return tprh::pipeline_tickets(nsys, __builtin_ctz((unsigned)(n / 4096)));
+}
+
+extern "C" int tprh_diag_split_clock(const float* a, const float* b, const float* c, const float* d, float* x, int n,
+                                     int nsys, void* workspace, int lag, unsigned long long* stamps, void* stream) {
+    using namespace tprh;
+    constexpr int E = 16, L = 16;
+    const int m = n / (E * L);
+    if (n % (E * L) || m * L < 2 * kPersistThreads || (m & (m - 1)) || m > kSplitMaxM || nsys < 1)
+        return (int)hipErrorInvalidValue;
+    SplitArgs p{};
+    p.n = n, p.nsys = nsys, p.chunk = nsys;
+    split_carve(workspace, m, nsys, &p);
+    BatchLoc<float> loc;
+    loc.a = a, loc.b = b, loc.c = c, loc.d = d, loc.x = x;
+    loc.rows = p.rows, loc.xed = p.xed, loc.n = n, loc.m = m, loc.cstride = (size_t)n;
+    const int bps = m * L / kPersistThreads;
+    const int lg_bps = __builtin_ctz((unsigned)bps);
+    const size_t lds = sizeof(float) * 10 * (size_t)m;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(split_clock_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * 10 * kSplitMaxM));
+    if (e != hipSuccess) return (int)e;
+    int dev = 0, cus = 0, occ = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return (int)e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return (int)e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, split_clock_kernel, kPersistThreads, lds)) !=
+        hipSuccess)
+        return (int)e;
+    const int cap = cus * occ;
+    if (cap < 1) return (int)hipErrorInvalidValue;
+    if (lag <= 0) lag = 2 * cap;
+    hipStream_t st = (hipStream_t)stream;
+    const int words = kSplitSyncStride * (nsys + 1);
+    if ((e = hipMemsetAsync(p.sync, 0, sizeof(int) * (size_t)words, st)) != hipSuccess) return (int)e;
+    SplitStampClock clk;
+    clk.out = stamps;
+    hipLaunchKernelGGL(split_clock_kernel, dim3(cap), dim3(kPersistThreads), lds, st, loc, p.sync, nsys, lg_bps, lag,
+                       clk);
     return (int)hipGetLastError();
 }
