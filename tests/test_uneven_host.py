@@ -81,3 +81,21 @@ def test_committed_schedule_is_valid():
         pytest.skip("no tuned schedule committed")
     for n, cand in autotune.schedule_of(autotune.load()).items():
         assert tuple(cand) in autotune.candidates(n)
+
+
+def test_class_choice_keeps_the_default_within_the_margin():
+    """A candidate replaces the default s = min(n, 256) shape only when it is
+    faster by more than the margin; the committed table follows the rule
+    from its own measurements."""
+    us = {"128/8x16": 98.0, "256/16x16": 100.0, "64/4x16": 130.0}
+    assert autotune.choose_class(1024, us) == [256, 16, 16]          # 2 % lead: noise
+    us["128/8x16"] = 90.0
+    assert autotune.choose_class(1024, us) == [128, 8, 16]           # 10 % lead
+    assert autotune.choose_class(1024, us, margin=0.2) == [256, 16, 16]
+    assert autotune.choose_class(128, {"128/8x16": 50.0, "64/4x16": 49.0}) == [128, 8, 16]
+    assert autotune.choose_class(64, {"64/4x16": 50.0, "32/2x16": 40.0}) == [32, 2, 16]
+    import os
+    if os.path.exists(autotune.DEFAULT_PATH):
+        table = autotune.load()
+        for n, row in table["classes"].items():
+            assert row["choice"] == autotune.choose_class(int(n), row["us"], table.get("margin", 0.0))

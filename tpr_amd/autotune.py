@@ -75,24 +75,43 @@ def _time_class(n, cand, method, eq_target, reps, warmup):
     return us, nsys
 
 
-def tune(sizes=None, method="ptpr", eq_target=16 << 20, reps=20, warmup=3, log=None):
+UNEVEN_MARGIN = 0.03
+
+
+def choose_class(n, us, margin=UNEVEN_MARGIN):
+    """The schedule entry for size class n from its measured times us
+    ({"s/ExL": microseconds}): the default (default_choice) unless another
+    candidate is faster by more than `margin` (a fraction).  Per-class times
+    within a few percent of each other do not carry over to a mixed batch in
+    one launch (on MI355X a table that took s = 128 for three classes on
+    1-2 % per-class leads ran 2-4 % slower on the 20000-system batch than
+    s = 256 throughout), so noise-level leads do not move the choice."""
+    best = min(us, key=lambda k: us[k])
+    dflt = "%d/%dx%d" % default_choice(n)
+    if dflt in us and us[dflt] <= us[best] * (1.0 + margin):
+        best = dflt
+    s, el = best.split("/")
+    E, L = el.split("x")
+    return [int(s), int(E), int(L)]
+
+
+def tune(sizes=None, method="ptpr", eq_target=16 << 20, reps=20, warmup=3, log=None, margin=UNEVEN_MARGIN):
     """Measure every candidate for each size class; returns the table dict
-    {"method", "classes": {n: {"choice": [s, E, L], "us": {...}, "gbps": ...}}}."""
+    {"method", "classes": {n: {"choice": [s, E, L], "us": {...}, "gbps": ...}}}
+    with each choice made by choose_class."""
     sizes = sizes or [1 << k for k in range(8, 15)]
-    res = {"method": method, "eq_per_class": eq_target, "classes": {}}
+    res = {"method": method, "eq_per_class": eq_target, "margin": margin, "classes": {}}
     for n in sizes:
         row = {"us": {}}
-        best = None
+        nsys = None
         for cand in candidates(n):
             us, nsys = _time_class(n, cand, method, eq_target, reps, warmup)
             key = "%d/%dx%d" % cand
             row["us"][key] = round(us, 2)
-            if best is None or us < best[0]:
-                best = (us, cand, nsys)
             if log:
                 log("n=%d %s %.2f us" % (n, key, us))
-        us, cand, nsys = best
-        row["choice"] = list(cand)
+        row["choice"] = choose_class(n, row["us"], margin)
+        us = row["us"]["%d/%dx%d" % tuple(row["choice"])]
         row["gbps"] = round(20.0 * n * nsys / (us * 1e-6) / 1e9, 1)
         res["classes"][str(n)] = row
     return res
