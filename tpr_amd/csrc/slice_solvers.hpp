@@ -505,6 +505,74 @@ __device__ __forceinline__ SliceEnd slice_end(const float* lds, int m, int j) {
     return SliceEnd{lds[j], lds[m + j], lds[2 * m + j]};
 }
 
+// reduced_solve<0> for a system whose m slices of L lanes lie in one
+// wavefront (m L <= 64), in its lanes base .. base + m L - 1: the same operations in
+// the same order (so the same bits), exchanged by cross-lane reads
+// (ds_bpermute) instead of LDS stores, barriers and loads -- a chain of five
+// exchanges instead of five store-barrier-load rounds, which shortens the
+// tail of a batch whose last wavefronts finish their loads last.  Every lane
+// gets x at the end of its slice (xe) and of the slice before (xprev, 0 for
+// the first) and its slice's E_ed (se).  Every lane of the wavefront calls
+// it.
+template <int L>
+__device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float eA, float eC, float eR, float sA,
+                                                   float sC, float sR, float& xe, float& xprev, SliceEnd& se) {
+    const int me = base + L * j;
+    const float ed_a = __shfl(eA, me + L - 1, 64), ed_c = __shfl(eC, me + L - 1, 64),
+                ed_r = __shfl(eR, me + L - 1, 64);
+    se = SliceEnd{ed_a, ed_c, ed_r};
+    const float s_a = __shfl(sA, me, 64), s_c = __shfl(sC, me, 64), s_r = __shfl(sR, me, 64);
+    // E_st += s1 * E_ed, src/ptpr.cpp:193-211
+    const float s1 = (fabsf(ed_c) < FLT_EPSILON) ? -1.0f : -s_c / ed_c;
+    const float st_a = s_a + s1 * ed_a, st_c = s1, st_r = s_r + s1 * ed_r;
+    // reduced row j from (E_ed(j), E_st(j+1)), src/ptpr.cpp:218-241
+    const int nx = base + L * (j + 1 < m ? j + 1 : j);
+    const float akr = __shfl(st_a, nx, 64), ckr = __shfl(st_c, nx, 64), rkr = __shfl(st_r, nx, 64);
+    float a, c, r;
+    if (j < m - 1) {
+        const float inv = 1.0f / (1.0f - akr * ed_c);
+        a = inv * ed_a;
+        c = -inv * ckr * ed_c;
+        r = inv * (ed_r - rkr * ed_c);
+    } else {
+        a = ed_a;
+        c = ed_c;
+        r = ed_r;
+    }
+    // PCR over the m rows, lds_pcr's operations (src/pcr.cpp:15-89)
+    for (int s = 1; s < m; s <<= 1) {
+        const int kl = base + L * (j >= s ? j - s : j);
+        const int kr = base + L * (j + s < m ? j + s : j);
+        const float al = __shfl(a, kl, 64), cl = __shfl(c, kl, 64), rl = __shfl(r, kl, 64);
+        const float ar = __shfl(a, kr, 64), cr = __shfl(c, kr, 64), rr = __shfl(r, kr, 64);
+        float e, na, nc, nr;
+        if (j < s) {
+            e = 1.0f / (1.0f - c * ar);
+            na = e * a;
+            nc = -e * c * cr;
+            nr = e * (r - c * rr);
+        } else if (j < m - s) {
+            const float ap = a, cp = c;
+            e = 1.0f / (1.0f - ap * cl - cp * ar);
+            na = -e * ap * al;
+            nc = -e * cp * cr;
+            nr = e * (r - ap * rl - cp * rr);
+        } else {
+            const float ap = a, cp = c;
+            e = 1.0f / (1.0f - ap * cl);
+            na = -e * ap * al;
+            nc = e * cp;
+            nr = e * (r - ap * rl);
+        }
+        a = na;
+        c = nc;
+        r = nr;
+    }
+    xe = r;
+    xprev = __shfl(r, base + L * (j > 0 ? j - 1 : 0), 64);
+    if (j == 0) xprev = 0.0f;
+}
+
 // ---------------------------------------------------------------------------
 // Stage entry points shared by the batched (one workgroup per system) and the
 // split (several workgroups per system) kernels.
@@ -577,9 +645,18 @@ __device__ __forceinline__ void ptpr_solve_regs(float (&A)[E], float (&C)[E], fl
     const int l = t % L;
     const int j = t / L;
     ptpr_stage1<E, L, MODE>(A, C, R, l);
-    const float* xr = reduced_solve<0, WAVE>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
-    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
-    ptpr_stage3<E, L, MODE>(A, C, R, xprev, xr[j], slice_end(lds, m, j), l, X);
+    if constexpr (WAVE) {
+        // the system's lanes are t = 0 .. m L - 1 of one wavefront
+        float xe, xprev;
+        SliceEnd se;
+        reduced_solve_wave<L>(m, j, (int)(threadIdx.x & 63) - t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0], xe,
+                              xprev, se);
+        ptpr_stage3<E, L, MODE>(A, C, R, xprev, xe, se, l, X);
+    } else {
+        const float* xr = reduced_solve<0, WAVE>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+        const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+        ptpr_stage3<E, L, MODE>(A, C, R, xprev, xr[j], slice_end(lds, m, j), l, X);
+    }
 }
 
 template <int E, int L, int MODE, bool WAVE = false>
@@ -714,9 +791,18 @@ __device__ __forceinline__ void ptpr_solve_fast16_partition(const RawPart<E>& p,
     const int j = t >> 4;
     Fast16State<E> st;
     fast16_stage1<E>(p, has_b, l, st);
-    const float* xr = reduced_solve<0, WAVE>(lds, m, j, l, 16, t, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR);
-    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
-    fast16_stage3<E>(st, xprev, xr[j], slice_end(lds, m, j), l, X);
+    if constexpr (WAVE) {
+        // the system's lanes are t = 0 .. 16 m - 1 of one wavefront
+        float xe, xprev;
+        SliceEnd se;
+        reduced_solve_wave<16>(m, j, (int)(threadIdx.x & 63) - t, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR, xe,
+                               xprev, se);
+        fast16_stage3<E>(st, xprev, xe, se, l, X);
+    } else {
+        const float* xr = reduced_solve<0, WAVE>(lds, m, j, l, 16, t, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR);
+        const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+        fast16_stage3<E>(st, xprev, xr[j], slice_end(lds, m, j), l, X);
+    }
 }
 
 // Whole PTPR solve from the raw loads (the division by the diagonal

@@ -2,7 +2,7 @@
 // solver library): the batched fast-PTPR solve of one-wavefront systems on
 // 16-lane slices (the shape of the 4096 x N=1024, S=256 batch), assembled
 // from the same device functions as batched_kernel -- load_raw_t,
-// fast16_stage1, reduced_solve, fast16_stage3, store_e -- with the GPU's
+// fast16_stage1, reduced_solve_wave, fast16_stage3, store_e -- with the GPU's
 // constant 100 MHz clock (s_memrealtime) read between them.  Each wave
 // records six stamps: start, loads complete, stage 1 done, reduced system
 // done, stage 3 done, stores complete.  The waits that make "loads complete"
@@ -67,10 +67,9 @@ __global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const flo
     keep(st.q);
     keep(st.eA), keep(st.eC), keep(st.eR), keep(st.sA), keep(st.sC), keep(st.sR);
     t[2] = stamp();
-    const float* xr = reduced_solve<0, true>(my, m, j, l, 16, tt, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR);
-    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
-    const float xe = xr[j];
-    const SliceEnd se = slice_end(my, m, j);
+    float xprev, xe;
+    SliceEnd se;
+    reduced_solve_wave<16>(m, j, (int)(threadIdx.x & 63) - tt, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR, xe, xprev, se);
     keep(xprev), keep(xe), keep(se.a), keep(se.c), keep(se.r);
     t[3] = stamp();
     float X[E];
