@@ -505,28 +505,30 @@ __device__ __forceinline__ SliceEnd slice_end(const float* lds, int m, int j) {
     return SliceEnd{lds[j], lds[m + j], lds[2 * m + j]};
 }
 
-// reduced_solve<0> for a system whose m slices of L lanes lie in one
-// wavefront (m L <= 64), in its lanes base .. base + m L - 1: the same operations in
-// the same order (so the same bits), exchanged by cross-lane reads
-// (ds_bpermute) instead of LDS stores, barriers and loads -- a chain of five
-// exchanges instead of five store-barrier-load rounds, which shortens the
-// tail of a batch whose last wavefronts finish their loads last.  Every lane
-// gets x at the end of its slice (xe) and of the slice before (xprev, 0 for
-// the first) and its slice's E_ed (se).  Every lane of the wavefront calls
-// it.
-template <int L>
+// reduced_solve for a system whose m slices of L lanes lie in one
+// wavefront (m L <= 64), in its lanes base .. base + m L - 1: the same
+// operations in the same order (so the same bits), exchanged by cross-lane
+// reads (ds_bpermute) instead of LDS stores, barriers and loads -- for
+// m = 4 a chain of five exchanges instead of five store-barrier-load rounds,
+// which shortens the tail of a batch whose last wavefronts finish their
+// loads last.  Row j of the reduced system lives in the lanes of slice j.
+// Every lane gets x at the end of its slice (xe) and of the slice before
+// (xprev, 0 for the first) and its slice's E_ed (se).  Every lane of the
+// wavefront calls it.
+template <int METHOD, int L>  // 0 = PTPR (PCR stage 2), 1 = TPR (CR stage 2)
 __device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float eA, float eC, float eR, float sA,
                                                    float sC, float sR, float& xe, float& xprev, SliceEnd& se) {
-    const int me = base + L * j;
+    auto lane = [&](int k) { return base + L * k; };  // a lane of slice k
+    const int me = lane(j);
     const float ed_a = __shfl(eA, me + L - 1, 64), ed_c = __shfl(eC, me + L - 1, 64),
                 ed_r = __shfl(eR, me + L - 1, 64);
     se = SliceEnd{ed_a, ed_c, ed_r};
     const float s_a = __shfl(sA, me, 64), s_c = __shfl(sC, me, 64), s_r = __shfl(sR, me, 64);
-    // E_st += s1 * E_ed, src/ptpr.cpp:193-211
+    // E_st += s1 * E_ed, src/ptpr.cpp:193-211 (src/tpr.cpp:327-340)
     const float s1 = (fabsf(ed_c) < FLT_EPSILON) ? -1.0f : -s_c / ed_c;
     const float st_a = s_a + s1 * ed_a, st_c = s1, st_r = s_r + s1 * ed_r;
     // reduced row j from (E_ed(j), E_st(j+1)), src/ptpr.cpp:218-241
-    const int nx = base + L * (j + 1 < m ? j + 1 : j);
+    const int nx = lane(j + 1 < m ? j + 1 : j);
     const float akr = __shfl(st_a, nx, 64), ckr = __shfl(st_c, nx, 64), rkr = __shfl(st_r, nx, 64);
     float a, c, r;
     if (j < m - 1) {
@@ -539,37 +541,89 @@ __device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float
         c = ed_c;
         r = ed_r;
     }
-    // PCR over the m rows, lds_pcr's operations (src/pcr.cpp:15-89)
-    for (int s = 1; s < m; s <<= 1) {
-        const int kl = base + L * (j >= s ? j - s : j);
-        const int kr = base + L * (j + s < m ? j + s : j);
-        const float al = __shfl(a, kl, 64), cl = __shfl(c, kl, 64), rl = __shfl(r, kl, 64);
-        const float ar = __shfl(a, kr, 64), cr = __shfl(c, kr, 64), rr = __shfl(r, kr, 64);
-        float e, na, nc, nr;
-        if (j < s) {
-            e = 1.0f / (1.0f - c * ar);
-            na = e * a;
-            nc = -e * c * cr;
-            nr = e * (r - c * rr);
-        } else if (j < m - s) {
-            const float ap = a, cp = c;
-            e = 1.0f / (1.0f - ap * cl - cp * ar);
-            na = -e * ap * al;
-            nc = -e * cp * cr;
-            nr = e * (r - ap * rl - cp * rr);
-        } else {
-            const float ap = a, cp = c;
-            e = 1.0f / (1.0f - ap * cl);
-            na = -e * ap * al;
-            nc = e * cp;
-            nr = e * (r - ap * rl);
+    if constexpr (METHOD == 0) {
+        // PCR over the m rows, lds_pcr's operations (src/pcr.cpp:15-89)
+        for (int s = 1; s < m; s <<= 1) {
+            const int kl = lane(j >= s ? j - s : j);
+            const int kr = lane(j + s < m ? j + s : j);
+            const float al = __shfl(a, kl, 64), cl = __shfl(c, kl, 64), rl = __shfl(r, kl, 64);
+            const float ar = __shfl(a, kr, 64), cr = __shfl(c, kr, 64), rr = __shfl(r, kr, 64);
+            float e, na, nc, nr;
+            if (j < s) {
+                e = 1.0f / (1.0f - c * ar);
+                na = e * a;
+                nc = -e * c * cr;
+                nr = e * (r - c * rr);
+            } else if (j < m - s) {
+                const float ap = a, cp = c;
+                e = 1.0f / (1.0f - ap * cl - cp * ar);
+                na = -e * ap * al;
+                nc = -e * cp * cr;
+                nr = e * (r - ap * rl - cp * rr);
+            } else {
+                const float ap = a, cp = c;
+                e = 1.0f / (1.0f - ap * cl);
+                na = -e * ap * al;
+                nc = e * cp;
+                nr = e * (r - ap * rl);
+            }
+            a = na;
+            c = nc;
+            r = nr;
         }
-        a = na;
-        c = nc;
-        r = nr;
+        xe = r;
+    } else {
+        // CR over the m rows, lds_cr's operations (src/cr.cpp:23-116)
+        if (m <= 1) {
+            xe = r;
+        } else {
+            // forward reduction: at level u, rows i = 2u-1, 4u-1, ... take
+            // rows i - u and i + u; the last row the lower formula
+            for (int u = 1; 2 * u < m; u <<= 1) {
+                const int kl = lane(j >= u ? j - u : j);
+                const int kr = lane(j + u < m ? j + u : j);
+                const float al = __shfl(a, kl, 64), cl = __shfl(c, kl, 64), rl = __shfl(r, kl, 64);
+                const float ar = __shfl(a, kr, 64), cr = __shfl(c, kr, 64), rr = __shfl(r, kr, 64);
+                if (((j + 1) & (2 * u - 1)) == 0) {
+                    float na, nc, nr;
+                    if (j < m - 2 * u) {
+                        const float inv = 1.0f / (1.0f - cl * a - ar * c);
+                        na = -inv * al * a;
+                        nc = -inv * cr * c;
+                        nr = inv * (r - rl * a - rr * c);
+                    } else {
+                        const float inv = 1.0f / (1.0f - cl * a);
+                        na = -inv * al * a;
+                        nc = inv * c;
+                        nr = inv * (r - rl * a);
+                    }
+                    a = na;
+                    c = nc;
+                    r = nr;
+                }
+            }
+            // backward substitution: the 2 x 2 system of rows m/2-1 and m-1,
+            // then each level's rows from the ones above it
+            const int h = m / 2;
+            const int top = lane(h - 1), bot = lane(m - 1);
+            const float c_t = __shfl(c, top, 64), r_t = __shfl(r, top, 64);
+            const float a_b = __shfl(a, bot, 64), r_b = __shfl(r, bot, 64);
+            const float inv_det = 1.0f / (1.0f - c_t * a_b);
+            float xo = 0.0f;
+            if (j == h - 1) xo = (r_t - c_t * r_b) * inv_det;
+            if (j == m - 1) xo = (r_b - r_t * a_b) * inv_det;
+            for (int u = h / 2; u >= 1; u >>= 1) {
+                const float xl = __shfl(xo, lane(j >= u ? j - u : j), 64);
+                const float xr = __shfl(xo, lane(j + u < m ? j + u : j), 64);
+                if (((j + 1) & (2 * u - 1)) == u) {  // rows u-1, 3u-1, ...
+                    if (j == u - 1) xo = r - c * xr;
+                    else if (j < m - u) xo = r - a * xl - c * xr;
+                }
+            }
+            xe = xo;
+        }
     }
-    xe = r;
-    xprev = __shfl(r, base + L * (j > 0 ? j - 1 : 0), 64);
+    xprev = __shfl(xe, lane(j > 0 ? j - 1 : 0), 64);
     if (j == 0) xprev = 0.0f;
 }
 
@@ -649,7 +703,7 @@ __device__ __forceinline__ void ptpr_solve_regs(float (&A)[E], float (&C)[E], fl
         // the system's lanes are t = 0 .. m L - 1 of one wavefront
         float xe, xprev;
         SliceEnd se;
-        reduced_solve_wave<L>(m, j, (int)(threadIdx.x & 63) - t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0], xe,
+        reduced_solve_wave<0, L>(m, j, (int)(threadIdx.x & 63) - t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0], xe,
                               xprev, se);
         ptpr_stage3<E, L, MODE>(A, C, R, xprev, xe, se, l, X);
     } else {
@@ -672,9 +726,18 @@ __device__ __forceinline__ void tpr_solve_regs(float (&A)[E], float (&C)[E], flo
         Ro[i] = R[i];
     }
     tpr_stage1_level<E, L, MODE, 1>(A, C, R, l);
-    const float* xr = reduced_solve<1, WAVE>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
-    const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
-    tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xr[j], l, X);
+    if constexpr (WAVE) {
+        // the system's lanes are t = 0 .. m L - 1 of one wavefront
+        float xe, xprev;
+        SliceEnd se;
+        reduced_solve_wave<1, L>(m, j, (int)(threadIdx.x & 63) - t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0], xe,
+                                 xprev, se);
+        tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xe, l, X);
+    } else {
+        const float* xr = reduced_solve<1, WAVE>(lds, m, j, l, L, t, A[E - 1], C[E - 1], R[E - 1], A[0], C[0], R[0]);
+        const float xprev = (j > 0) ? xr[j - 1] : 0.0f;
+        tpr_stage3<E, L, MODE>(A, C, R, Ao, Co, Ro, xprev, xr[j], l, X);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -795,7 +858,7 @@ __device__ __forceinline__ void ptpr_solve_fast16_partition(const RawPart<E>& p,
         // the system's lanes are t = 0 .. 16 m - 1 of one wavefront
         float xe, xprev;
         SliceEnd se;
-        reduced_solve_wave<16>(m, j, (int)(threadIdx.x & 63) - t, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR, xe,
+        reduced_solve_wave<0, 16>(m, j, (int)(threadIdx.x & 63) - t, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR, xe,
                                xprev, se);
         fast16_stage3<E>(st, xprev, xe, se, l, X);
     } else {

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const flo
     t[2] = stamp();
     float xprev, xe;
     SliceEnd se;
-    reduced_solve_wave<16>(m, j, (int)(threadIdx.x & 63) - tt, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR, xe, xprev, se);
+    reduced_solve_wave<0, 16>(m, j, (int)(threadIdx.x & 63) - tt, st.eA, st.eC, st.eR, st.sA, st.sC, st.sR, xe, xprev, se);
     keep(xprev), keep(xe), keep(se.a), keep(se.c), keep(se.r);
     t[3] = stamp();
     float X[E];
