@@ -157,6 +157,39 @@ struct Slot {
     double predicted = 0;   // its expected time on the device, s (adaptive window)
 };
 
+// The server's lock.  Client calls and the worker each hold it for a few
+// microseconds; a thread that finds it taken spins on try_lock for a while
+// before it sleeps.  With std::mutex alone a contended call slept in the
+// kernel at once and paid a futex wake-up of several microseconds, several
+// times per request (submit, wait, info, release from the client threads,
+// admission and retirement from the worker), which bounded the closed-loop
+// request rate.
+class ServerMutex {
+  public:
+    void lock() {
+        for (int i = 0; i < kSpins; i++) {
+            if (m_.try_lock()) return;
+            relax();
+        }
+        m_.lock();
+    }
+    bool try_lock() { return m_.try_lock(); }
+    void unlock() { m_.unlock(); }
+
+  private:
+    static constexpr int kSpins = 2000;
+    static void relax() {
+#if defined(__x86_64__) && !defined(__HIP_DEVICE_COMPILE__)
+        __builtin_ia32_pause();
+#else
+        std::this_thread::yield();
+#endif
+    }
+    std::mutex m_;
+};
+using ServerLock = std::unique_lock<ServerMutex>;
+using ServerGuard = std::lock_guard<ServerMutex>;
+
 }  // namespace
 
 struct tprh_server {
@@ -169,8 +202,8 @@ struct tprh_server {
     Arena arena{0};
     std::chrono::steady_clock::time_point t0;
 
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    ServerMutex mu;
+    std::condition_variable_any cv_work, cv_done;
     std::deque<long long> queue;
     long long queued_systems = 0;  // right-hand sides of the queued requests
     // bumped (under mu) by every call that may let a queued request in: a
@@ -519,7 +552,7 @@ void worker_main(tprh_server* sv) {
             std::this_thread::yield();
             continue;
         }
-        std::unique_lock<std::mutex> lk(sv->mu);
+        ServerLock lk(sv->mu);
         seen = wake;
         recheck = false;
         for (const auto& f : fired) {
@@ -775,7 +808,7 @@ tprh_status tprh_server_create(tprh_server** out, const tprh_server_config* cfg)
 tprh_status tprh_server_destroy(tprh_server* sv) {
     if (!sv) return TPRH_SUCCESS;
     {
-        std::lock_guard<std::mutex> g(sv->mu);
+        ServerGuard g(sv->mu);
         sv->stop = true;
         sv->wake++;
     }
@@ -803,7 +836,7 @@ tprh_status tprh_server_destroy(tprh_server* sv) {
 // the record.  A generated one is filled by the first batch that uses it.
 static tprh_status new_matrix(tprh_server* sv, int n, uint64_t sys, hipEvent_t ev, int gen_kind, uint64_t gen_seed,
                               long long* id, size_t* off_out) {
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     int s, E, L;
     bool split;
     tprh_status st = validate_n(sv, n, &s, &E, &L, &split);
@@ -849,11 +882,11 @@ tprh_status tprh_server_matrix_upload(tprh_server* sv, int n, const float* a, co
         (e = hipMemcpyAsync(dst + 2 * (size_t)n, c, nb, hipMemcpyDefault, sv->upload)) != hipSuccess ||
         (e = hipEventRecord(ev, sv->upload)) != hipSuccess || (e = hipEventSynchronize(ev)) != hipSuccess) {
         (void)hipStreamSynchronize(sv->upload);  // copies already queued finish before the block is reused
-        std::lock_guard<std::mutex> g(sv->mu);
+        ServerGuard g(sv->mu);
         free_matrix_locked(sv, *matrix);
         return tprh::set_hip_error(e, "matrix upload");
     }
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     sv->mats[*matrix].ready_known = true;
     return TPRH_SUCCESS;
 }
@@ -869,7 +902,7 @@ tprh_status tprh_server_matrix_generate(tprh_server* sv, int n, tprh_gen kind, u
 
 tprh_status tprh_server_matrix_release(tprh_server* sv, long long matrix) {
     if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     auto it = sv->mats.find(matrix);
     if (it == sv->mats.end() || it->second.released)
         return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown matrix");
@@ -885,7 +918,7 @@ tprh_status tprh_server_matrix_release(tprh_server* sv, long long matrix) {
 tprh_status tprh_server_submit(tprh_server* sv, long long matrix, int nrhs, const float* d, uint64_t rhs_seed, int j0,
                                float* x, long long* request) {
     if (!sv || !request) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
-    std::unique_lock<std::mutex> lk(sv->mu);
+    ServerLock lk(sv->mu);
     auto it = sv->mats.find(matrix);
     if (it == sv->mats.end() || it->second.released)
         return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown matrix");
@@ -923,7 +956,7 @@ tprh_status tprh_server_submit(tprh_server* sv, long long matrix, int nrhs, cons
 
 tprh_status tprh_server_request_info(tprh_server* sv, long long request, tprh_request_info* info) {
     if (!sv || !info) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     auto it = sv->reqs.find(request);
     if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
     const Request& r = it->second;
@@ -940,7 +973,7 @@ tprh_status tprh_server_request_info(tprh_server* sv, long long request, tprh_re
 tprh_status tprh_server_wait(tprh_server* sv, long long request, double timeout_s) {
     if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
     if (!(timeout_s >= 0.0)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "timeout must be >= 0");
-    std::unique_lock<std::mutex> lk(sv->mu);
+    ServerLock lk(sv->mu);
     const auto deadline = wait_deadline(timeout_s);
     for (;;) {
         auto it = sv->reqs.find(request);
@@ -974,7 +1007,7 @@ static void take_completed_locked(tprh_server* sv, long long* ids, int max, int*
 
 tprh_status tprh_server_poll(tprh_server* sv, long long* ids, int max, int* count) {
     if (!sv || !count || (max > 0 && !ids)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     take_completed_locked(sv, ids, max, count);
     return TPRH_SUCCESS;
 }
@@ -982,7 +1015,7 @@ tprh_status tprh_server_poll(tprh_server* sv, long long* ids, int max, int* coun
 tprh_status tprh_server_wait_any(tprh_server* sv, long long* ids, int max, int* count, double timeout_s) {
     if (!sv || !count || max < 1 || !ids) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument or max < 1");
     if (!(timeout_s >= 0.0)) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "timeout must be >= 0");
-    std::unique_lock<std::mutex> lk(sv->mu);
+    ServerLock lk(sv->mu);
     const auto deadline = wait_deadline(timeout_s);
     *count = 0;
     // the list may hold only ids released meanwhile: then wait on
@@ -997,7 +1030,7 @@ tprh_status tprh_server_copy_result(tprh_server* sv, long long request, float* d
     if (!sv || !dst) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
     // under the lock, so that a concurrent tprh_server_request_release cannot
     // free the block while it is copied
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     auto it = sv->reqs.find(request);
     if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
     const Request& r = it->second;
@@ -1012,7 +1045,7 @@ tprh_status tprh_server_copy_result(tprh_server* sv, long long request, float* d
 
 tprh_status tprh_server_request_release(tprh_server* sv, long long request) {
     if (!sv) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null server");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     auto it = sv->reqs.find(request);
     if (it == sv->reqs.end()) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "unknown request");
     Request& r = it->second;
@@ -1038,7 +1071,7 @@ double tprh_server_now(const tprh_server* sv) { return sv ? sv->now() : 0.0; }
 
 tprh_status tprh_server_stats_get(tprh_server* sv, tprh_server_stats* out) {
     if (!sv || !out) return tprh::set_error(TPRH_ERROR_INVALID_VALUE, "null argument");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ServerGuard g(sv->mu);
     *out = sv->st;
     out->queued = (int)sv->queue.size();
     out->bytes_in_use = sv->arena.used();
