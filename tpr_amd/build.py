@@ -32,7 +32,7 @@ NSTAB_BIN = os.path.join(HERE, "tpr_nstab")
 SERVE_SRC = os.path.join(HERE, "cli", "tpr_serve_bench.cpp")
 SERVE_BIN = os.path.join(HERE, "tpr_serve_bench")
 VENDOR_SRC = os.path.join(HERE, "comparators", "rocsparse_gtsv.cpp")
-DIAG_SRC = os.path.join(HERE, "diag", "stage_clock.hip")
+DIAG_SRC = [os.path.join(HERE, "diag", "stage_clock.hip"), os.path.join(HERE, "diag", "exact_div.hip")]
 DIAG_LIB = os.path.join(HERE, "libtpr_diag.so")
 VENDOR_LIB = os.path.join(HERE, "libtpr_rocsparse.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -97,11 +97,13 @@ def build_vendor():
 
 
 def build_diag():
-    """libtpr_diag.so: the per-stage clock build of the batched fast-PTPR
-    solve (tpr_amd/diag/stage_clock.hip, run by tpr_amd.stage_clock); a
-    diagnostic, not part of the solver library."""
+    """libtpr_diag.so: the per-stage clock builds of the batched fast-PTPR
+    solve and of the split path's pipelined form (tpr_amd/diag/stage_clock.hip,
+    run by tpr_amd.stage_clock) and the exhaustive check of the exact
+    kernels' reciprocal (tpr_amd/diag/exact_div.hip); a diagnostic, not part
+    of the solver library."""
     tmp = DIAG_LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-shared", DIAG_SRC, "-o", tmp])
+    _run([HIPCC] + HIP_FLAGS + ["-shared"] + DIAG_SRC + ["-o", tmp])
     os.replace(tmp, DIAG_LIB)
     return DIAG_LIB
 

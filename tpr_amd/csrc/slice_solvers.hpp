@@ -477,7 +477,7 @@ __device__ __forceinline__ const float* reduced_solve(float* lds, int m, int j, 
         if (t < m - 1) {
             float ak = ed_a[t], ck = ed_c[t], rk = ed_r[t];
             float akr = st_a[t + 1], ckr = st_c[t + 1], rkr = st_r[t + 1];
-            float inv = 1.0f / (1.0f - akr * ck);
+            float inv = recip_rn(1.0f - akr * ck);
             ra0[t] = inv * ak;
             rc0[t] = -inv * ckr * ck;
             rr0[t] = inv * (rk - rkr * ck);
@@ -532,7 +532,7 @@ __device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float
     const float akr = __shfl(st_a, nx, 64), ckr = __shfl(st_c, nx, 64), rkr = __shfl(st_r, nx, 64);
     float a, c, r;
     if (j < m - 1) {
-        const float inv = 1.0f / (1.0f - akr * ed_c);
+        const float inv = recip_rn(1.0f - akr * ed_c);
         a = inv * ed_a;
         c = -inv * ckr * ed_c;
         r = inv * (ed_r - rkr * ed_c);
@@ -550,19 +550,19 @@ __device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float
             const float ar = __shfl(a, kr, 64), cr = __shfl(c, kr, 64), rr = __shfl(r, kr, 64);
             float e, na, nc, nr;
             if (j < s) {
-                e = 1.0f / (1.0f - c * ar);
+                e = recip_rn(1.0f - c * ar);
                 na = e * a;
                 nc = -e * c * cr;
                 nr = e * (r - c * rr);
             } else if (j < m - s) {
                 const float ap = a, cp = c;
-                e = 1.0f / (1.0f - ap * cl - cp * ar);
+                e = recip_rn(1.0f - ap * cl - cp * ar);
                 na = -e * ap * al;
                 nc = -e * cp * cr;
                 nr = e * (r - ap * rl - cp * rr);
             } else {
                 const float ap = a, cp = c;
-                e = 1.0f / (1.0f - ap * cl);
+                e = recip_rn(1.0f - ap * cl);
                 na = -e * ap * al;
                 nc = e * cp;
                 nr = e * (r - ap * rl);
@@ -587,12 +587,12 @@ __device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float
                 if (((j + 1) & (2 * u - 1)) == 0) {
                     float na, nc, nr;
                     if (j < m - 2 * u) {
-                        const float inv = 1.0f / (1.0f - cl * a - ar * c);
+                        const float inv = recip_rn(1.0f - cl * a - ar * c);
                         na = -inv * al * a;
                         nc = -inv * cr * c;
                         nr = inv * (r - rl * a - rr * c);
                     } else {
-                        const float inv = 1.0f / (1.0f - cl * a);
+                        const float inv = recip_rn(1.0f - cl * a);
                         na = -inv * al * a;
                         nc = inv * c;
                         nr = inv * (r - rl * a);
@@ -608,7 +608,7 @@ __device__ __forceinline__ void reduced_solve_wave(int m, int j, int base, float
             const int top = lane(h - 1), bot = lane(m - 1);
             const float c_t = __shfl(c, top, 64), r_t = __shfl(r, top, 64);
             const float a_b = __shfl(a, bot, 64), r_b = __shfl(r, bot, 64);
-            const float inv_det = 1.0f / (1.0f - c_t * a_b);
+            const float inv_det = recip_rn(1.0f - c_t * a_b);
             float xo = 0.0f;
             if (j == h - 1) xo = (r_t - c_t * r_b) * inv_det;
             if (j == m - 1) xo = (r_b - r_t * a_b) * inv_det;
@@ -659,7 +659,7 @@ __device__ __forceinline__ void ptpr_stage3(const float (&A)[E], const float (&C
                                             float xprev, float xe, SliceEnd se, int l, float (&X)[E]) {
     float key;
     if constexpr (MODE == kExact) {
-        key = 1.0f / se.c * (se.r - se.a * xprev - xe);
+        key = recip_rn(se.c) * (se.r - se.a * xprev - xe);
     } else {
         key = fast_rcp(se.c) * (__builtin_fmaf(-se.a, xprev, se.r) - xe);
     }

@@ -151,7 +151,7 @@ __device__ __forceinline__ const float* reduced_rows_lds(const float* sysrows, i
         const float ak = ra1[j], ck = rc1[j], rk = rr1[j];
         if (j < m - 1) {
             const float akr = st_a[j + 1], ckr = st_c[j + 1], rkr = st_r[j + 1];
-            const float inv = 1.0f / (1.0f - akr * ck);
+            const float inv = recip_rn(1.0f - akr * ck);
             ra0[j] = inv * ak;
             rc0[j] = -inv * ckr * ck;
             rr0[j] = inv * (rk - rkr * ck);

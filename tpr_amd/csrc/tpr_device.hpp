@@ -82,6 +82,29 @@ __device__ __forceinline__ float shift_down(float v, float bnd, int l) {
 // Reciprocal used by the Fast policy: v_rcp_f32 (1 ulp).
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// 1 / x rounded to nearest, the same bits as IEEE division (1.0f / x) gives,
+// in four instructions instead of eleven: the hardware reciprocal (within an
+// ulp), one Newton step in FMA, and v_div_fixup for zeros, infinities and
+// NaNs.  Checked against 1.0f / x for every one of the 2^32 float inputs by
+// tests/test_gpu_exact_recip.py (tpr_amd/diag/exact_div.hip); inputs whose
+// reciprocal leaves the normal range (|x| < 2^-126 or |x| > 2^126) take the
+// IEEE division, behind a branch the wavefront skips when no lane needs it.
+__device__ __forceinline__ float recip_rn(float x) {
+    const unsigned ax = __builtin_bit_cast(unsigned, x) & 0x7fffffffu;
+    if (ax - 0x00800000u >= 0x7e800000u - 0x00800000u) return 1.0f / x;
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_amdgcn_div_fixupf(__builtin_fmaf(e, r, r), x, 1.0f);
+}
+
+// The same without the range branch, for the exhaustive check of where the
+// short sequence alone is exact.
+__device__ __forceinline__ float recip_rn_unguarded(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_amdgcn_div_fixupf(__builtin_fmaf(e, r, r), x, 1.0f);
+}
+
 // ---------------------------------------------------------------------------
 // Fused DPP forms for slices of L == 16 lanes (one DPP row), Fast policy.
 //
@@ -170,7 +193,7 @@ __device__ __forceinline__ Eq pcr_update(Eq k, Eq kl, Eq kr, bool hasL, bool has
         // inv = 1 / (1 - c[kl]*a[k] - a[kr]*c[k])
         float t1 = hasL ? (1.0f - kl.c * k.a) : 1.0f;
         float den = hasR ? (t1 - kr.a * k.c) : t1;
-        float inv = 1.0f / den;
+        float inv = recip_rn(den);
         o.a = hasL ? (-inv * kl.a) * k.a : inv * k.a;
         o.c = hasR ? (-inv * kr.c) * k.c : inv * k.c;
         float rn = hasL ? (k.r - kl.r * k.a) : k.r;
@@ -224,21 +247,21 @@ __device__ __forceinline__ float* lds_pcr(float* a, float* c, float* r, float* a
             float e, na, nc, nr;
             if (k < s) {
                 int kr = k + s;
-                e = 1.0f / (1.0f - c[k] * a[kr]);
+                e = recip_rn(1.0f - c[k] * a[kr]);
                 na = e * a[k];
                 nc = -e * c[k] * c[kr];
                 nr = e * (r[k] - c[k] * r[kr]);
             } else if (k < m - s) {
                 int kl = k - s, kr = k + s;
                 float ap = a[k], cp = c[k];
-                e = 1.0f / (1.0f - ap * c[kl] - cp * a[kr]);
+                e = recip_rn(1.0f - ap * c[kl] - cp * a[kr]);
                 na = -e * ap * a[kl];
                 nc = -e * cp * c[kr];
                 nr = e * (r[k] - ap * r[kl] - cp * r[kr]);
             } else {
                 int kl = k - s;
                 float ap = a[k], cp = c[k];
-                e = 1.0f / (1.0f - ap * c[kl]);
+                e = recip_rn(1.0f - ap * c[kl]);
                 na = -e * ap * a[kl];
                 nc = e * cp;
                 nr = e * (r[k] - ap * r[kl]);
@@ -285,13 +308,13 @@ __device__ __forceinline__ void lds_cr(float* a, float* c, float* r, float* xo, 
             if (act) {
                 if (i < m - ux) {
                     int kl = i - u, k = i, kr = i + u;
-                    float inv = 1.0f / (1.0f - c[kl] * a[k] - a[kr] * c[k]);
+                    float inv = recip_rn(1.0f - c[kl] * a[k] - a[kr] * c[k]);
                     na = -inv * a[kl] * a[k];
                     nc = -inv * c[kr] * c[k];
                     nr = inv * (r[k] - r[kl] * a[k] - r[kr] * c[k]);
                 } else {
                     int k = m - 1, kl = k - u;
-                    float inv = 1.0f / (1.0f - c[kl] * a[k]);
+                    float inv = recip_rn(1.0f - c[kl] * a[k]);
                     na = -inv * a[kl] * a[k];
                     nc = inv * c[k];
                     nr = inv * (r[k] - r[kl] * a[k]);
@@ -309,7 +332,7 @@ __device__ __forceinline__ void lds_cr(float* a, float* c, float* r, float* xo, 
     // backward substitution, src/cr.cpp:88-116
     if (t == 0) {
         int i = m / 2 - 1, u = m / 2;
-        float inv_det = 1.0f / (1.0f - c[i] * a[i + u]);
+        float inv_det = recip_rn(1.0f - c[i] * a[i + u]);
         xo[i] = (r[i] - c[i] * r[i + u]) * inv_det;
         xo[i + u] = (r[i + u] - r[i] * a[i + u]) * inv_det;
     }
