@@ -108,9 +108,14 @@ hipError_t launch_el(const BatchArgs& p, hipStream_t stream) {
     const int spb = block / T;
     const int grid = (p.nsys + spb - 1) / spb;
     const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(p.m);
-    // the wavefront-synchronised variant only when a system fits one wavefront
-    hipError_t e = T <= 64 ? launch_variant<METHOD, E, L, MODE, kSmallBlock>(p, grid, block, shmem, lg, stream)
-                           : launch_variant<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
+    // the wavefront-synchronised variant only when a system fits one
+    // wavefront; systems of up to 512 lanes take a kernel bounded to 512
+    // lanes per workgroup, whose waves may hold up to 256 VGPRs (a bound of
+    // 1024 lanes caps them at 128, and exact arithmetic on 16 equations per
+    // lane then spills)
+    hipError_t e = T <= 64    ? launch_variant<METHOD, E, L, MODE, kSmallBlock>(p, grid, block, shmem, lg, stream)
+                   : T <= 512 ? launch_variant<METHOD, E, L, MODE, 512>(p, grid, block, shmem, lg, stream)
+                              : launch_variant<METHOD, E, L, MODE, 1024>(p, grid, block, shmem, lg, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -1,0 +1,11 @@
+// split_ptpr_fast.hip -- the fast-arithmetic PTPR instantiations of the split path
+// (split_impl.hpp).
+#include "split_impl.hpp"
+
+namespace tprh {
+
+hipError_t launch_split_ptpr_fast(const SplitArgs& p, int E, int L, hipStream_t stream) {
+    return launch_split_mode<0, kFast>(p, E, L, stream);
+}
+
+}  // namespace tprh
