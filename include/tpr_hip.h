@@ -309,7 +309,12 @@ tprh_status tprh_server_matrix_release(tprh_server *srv, long long matrix);
  * the answers go (nrhs * n floats, host or device) when the request
  * completes, or NULL to keep them in the arena until
  * tprh_server_request_release (tprh_server_request_info gives the device
- * address).  TPRH_ERROR_BUDGET: the request needs more than the whole arena. */
+ * address).  TPRH_ERROR_BUDGET: the request needs more than the whole arena.
+ * Host d and x should be pinned (hipHostMalloc) or device memory: both are
+ * copied by hipMemcpyAsync on the batch's stream (d before its kernels, x
+ * after them), and a copy from or to pageable host memory may make the
+ * server's worker thread wait for that batch, so that batches no longer
+ * overlap.  Pageable buffers give the same answers. */
 tprh_status tprh_server_submit(tprh_server *srv, long long matrix, int nrhs, const float *d, uint64_t rhs_seed,
                                int j0, float *x, long long *request);
 
