@@ -269,16 +269,17 @@ typedef struct tprh_server_config {
      * max_wait_us (0, 0: launch whenever a slot is free). */
     int min_batch_systems;
     double max_wait_us;
-    /* 1: adaptive window instead of the fixed one.  While batches run, the
-     * next batch waits until the earliest of them is expected to finish
-     * within the time the server takes to build and launch a batch (device
-     * time per equation and launch time are measured as batches complete),
-     * or until it holds max_batch_systems right-hand sides, or (when
-     * max_wait_us > 0) until its oldest request has waited max_wait_us;
-     * min_batch_systems is not used.  The device stays busy and each batch
-     * takes what arrived while the others ran.  It helps where the device
-     * is the bottleneck; a server bound by host work per request (small
-     * batches that finish before the next is formed) sees no change.  0:
+    /* 1: adaptive window instead of the fixed one.  The next batch waits
+     * (a) while requests arrive faster than one per launch time (the time
+     * the server takes to build and launch a batch, measured), until it
+     * holds about twice the requests that arrive in one launch time -- so
+     * that under heavy load batches stay large and the worker is not the
+     * bottleneck; and (b) while batches run, until the earliest of them is
+     * expected to finish within one launch time (device time per equation
+     * is measured as batches complete).  It never waits past
+     * max_batch_systems right-hand sides, nor past max_wait_us for its
+     * oldest request (four launch times when max_wait_us is 0);
+     * min_batch_systems is not used.  Under light load nothing waits.  0:
      * the fixed window above. */
     int adaptive_window;
 } tprh_server_config;

@@ -228,6 +228,9 @@ struct tprh_server {
     // exponentially weighted averages over the batches seen so far
     double s_per_eq = 0.0;
     double launch_s = 50e-6;
+    // and (under mu, from the submits) time between request arrivals
+    double arr_gap = 0.0;
+    double t_last_submit = -1.0;
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -508,13 +511,25 @@ hipError_t launch_batch(tprh_server* sv, Slot& sl, const std::vector<Staged>& it
 // has waited max_wait_us.  Expected finish times come from the measured
 // device time per equation of the batches seen so far; before the first one
 // has completed nothing waits.
-bool batch_should_wait(tprh_server* sv) {
+bool batch_should_wait(tprh_server* sv, int inflight) {
     const double now = sv->now();
     const double waited_us = 1e6 * (now - sv->reqs[sv->queue.front()].t_submit);
     if (!sv->cfg.adaptive_window)
-        return sv->queued_systems < sv->cfg.min_batch_systems && waited_us < sv->cfg.max_wait_us;
-    if (sv->s_per_eq <= 0.0 || sv->queued_systems >= sv->cfg.max_batch_systems) return false;
-    if (sv->cfg.max_wait_us > 0.0 && waited_us >= sv->cfg.max_wait_us) return false;
+        return inflight > 0 && sv->queued_systems < sv->cfg.min_batch_systems && waited_us < sv->cfg.max_wait_us;
+    if (sv->queued_systems >= sv->cfg.max_batch_systems) return false;
+    const double cap_us = sv->cfg.max_wait_us > 0.0 ? sv->cfg.max_wait_us : 4e6 * sv->launch_s;
+    if (waited_us >= cap_us) return false;
+    // Arrivals faster than one per launch: the worker, not the device, sets
+    // the pace, and a batch of whatever is queued stays small (it finishes
+    // before the next decision, so nothing is in flight to wait for).  Hold
+    // the batch until it has about twice the requests that arrive during
+    // one launch, so the worker spends at most about half its time
+    // launching.  At light load (fewer than one arrival per launch) this
+    // never applies, and requests go at once.
+    if (sv->arr_gap > 0.0 && sv->launch_s >= sv->arr_gap &&
+        (double)sv->queue.size() < 2.0 * sv->launch_s / sv->arr_gap)
+        return true;
+    if (inflight == 0 || sv->s_per_eq <= 0.0) return false;
     double first_free = 1e300;
     for (const Slot& sl : sv->slots)
         if (sl.busy) first_free = std::min(first_free, sl.t_launched + sl.predicted);
@@ -566,12 +581,13 @@ void worker_main(tprh_server* sv) {
         }
         if (!fired.empty()) sv->cv_done.notify_all();
         // 2. admission and launch
-        bool launched = false;
+        bool launched = false, windowed = false;
         for (Slot& sl : sv->slots) {
             if (sl.busy || sv->queue.empty()) continue;
-            if (inflight > 0 && batch_should_wait(sv)) {
+            if (batch_should_wait(sv, inflight)) {
                 recheck = true;
-                break;  // batches are running: let this one grow
+                windowed = true;
+                break;  // let this batch grow
             }
             std::vector<long long> batch;
             long long systems = 0, equations = 0;
@@ -664,7 +680,9 @@ void worker_main(tprh_server* sv) {
         if (sv->stop && sv->queue.empty() && inflight == 0) break;
         // 3. idle: sleep until a submit, a release or the destructor
         if (inflight == 0 && !launched) {
-            sv->cv_work.wait_for(lk, std::chrono::milliseconds(50));
+            // a batch held by the window is looked at again soon
+            if (windowed) sv->cv_work.wait_for(lk, std::chrono::microseconds(10));
+            else sv->cv_work.wait_for(lk, std::chrono::milliseconds(50));
             recheck = true;
         }
     }
@@ -941,6 +959,11 @@ tprh_status tprh_server_submit(tprh_server* sv, long long matrix, int nrhs, cons
     r.rhs_seed = rhs_seed;
     r.j0 = j0;
     r.t_submit = sv->now();
+    if (sv->t_last_submit >= 0.0) {
+        const double gap = r.t_submit - sv->t_last_submit;
+        sv->arr_gap = sv->arr_gap > 0.0 ? 0.9 * sv->arr_gap + 0.1 * gap : gap;
+    }
+    sv->t_last_submit = r.t_submit;
     const long long id = sv->next_req++;
     sv->reqs[id] = r;
     sv->queue.push_back(id);
