@@ -396,10 +396,10 @@ def copy_workload(args):
 PREC_NSYS, PREC_N, PREC_S = 256, 65536, 256
 
 
-def graph_ms(fn, reps):
+def graph_ms(fn, reps, samples=3):
     """Device time per call of fn: `reps` calls captured in one HIP graph,
-    replayed once untimed and once between HIP events on the current
-    stream."""
+    replayed once untimed, then `samples` times between HIP events on the
+    current stream; the median of the samples."""
     import torch
     fn()
     torch.cuda.synchronize()
@@ -409,13 +409,16 @@ def graph_ms(fn, reps):
             fn()
     g.replay()
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    g.replay()
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    times = []
+    for _ in range(samples):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / reps)
+    return sorted(times)[len(times) // 2]
 
 
 def precision_setup(chunk=0, split_form=0):
