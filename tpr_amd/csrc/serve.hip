@@ -20,12 +20,14 @@ __global__ __launch_bounds__(1024) void serve_kernel(const UnevenTask* __restric
                                                      float* __restrict__ base) {
     extern __shared__ float lds[];
     __shared__ int item;
-    for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = item;
-        __syncthreads();  // every thread has read `item` before it is rewritten
-        if (it >= ntasks) break;  // uniform: every workgroup leaves once the items are gone
+    if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+    __syncthreads();
+    int it = item;
+    while (it < ntasks) {  // uniform: every workgroup leaves once the items are gone
+        // the next ticket is asked for now and read after this item, so the
+        // device-coherent counter's latency hides under the item's loads
+        int nxt = ntasks;
+        if (threadIdx.x == 0) nxt = atomicAdd(counter, 1);
         const UnevenTask tk = tasks[it];
         const int slot = threadIdx.x >> tk.lg_lanes;
         const bool valid = slot < tk.count;
@@ -44,7 +46,11 @@ __global__ __launch_bounds__(1024) void serve_kernel(const UnevenTask* __restric
             default:
                 break;
         }
-        __syncthreads();  // the item's LDS rows are read before the next item writes them
+        __syncthreads();  // the item's LDS rows are read before the next item writes them;
+                          // every thread has read `item` before it is rewritten
+        if (threadIdx.x == 0) item = nxt;
+        __syncthreads();
+        it = item;
     }
 }
 

@@ -6,7 +6,9 @@
 // as many systems as one 1024-lane workgroup holds.  The persistent kernel
 // keeps one workgroup per resident slot; each workgroup takes the next work
 // item from an atomic counter (items are ordered largest systems first) and
-// solves it with the shape's code path, until the items run out.  Every
+// solves it with the shape's code path, until the items run out; it asks
+// for its next ticket as it starts an item, so the counter's latency hides
+// under the item's loads.  Every
 // system's result depends only on its own size class's (s, E, L), never on
 // the batch around it.
 #include <hip/hip_runtime.h>
@@ -41,12 +43,16 @@ __global__ __launch_bounds__(1024) void uneven_kernel(const UnevenTask* __restri
                                                       float* __restrict__ x) {
     extern __shared__ float lds[];
     __shared__ int item;
-    for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = item;
-        __syncthreads();  // every thread has read `item` before it is rewritten
-        if (it >= ntasks) break;  // uniform: every workgroup leaves once the items are gone
+    if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+    __syncthreads();
+    int it = item;
+    while (it < ntasks) {  // uniform: every workgroup leaves once the items are gone
+        // the next item's ticket is asked for now and read after this item:
+        // the counter is device-coherent, its answer takes microseconds
+        // behind the memory traffic, and this item's loads now cover that
+        // wait (a workgroup holds at most two tickets; items are independent)
+        int nxt = ntasks;
+        if (threadIdx.x == 0) nxt = atomicAdd(counter, 1);
         const UnevenTask tk = tasks[it];
         switch (tk.shape) {
 #define TPRH_UNEVEN_CASE(ID, EE, LL)                                                                     \
@@ -58,7 +64,11 @@ __global__ __launch_bounds__(1024) void uneven_kernel(const UnevenTask* __restri
             default:
                 break;
         }
-        __syncthreads();  // the item's LDS rows are read before the next item writes them
+        __syncthreads();  // the item's LDS rows are read before the next item writes them;
+                          // every thread has read `item` before it is rewritten
+        if (threadIdx.x == 0) item = nxt;
+        __syncthreads();
+        it = item;
     }
 }
 
