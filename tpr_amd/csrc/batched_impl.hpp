@@ -61,7 +61,7 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
 #pragma unroll
         for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
     }
-    if (valid) store_e<E>(x + off, X);
+    if (valid) store_e<E, true>(x + off, X);  // non-temporal: x is not read again here
 }
 
 namespace {

@@ -49,9 +49,21 @@ __device__ __forceinline__ void load_e(const float* __restrict__ p, float (&v)[E
     }
 }
 
-template <int E>
+// NT: non-temporal stores (marked streaming for the caches).  On MI355X
+// they make the one-workgroup batched kernel's x stores cheaper (4096 x
+// N=1024: PTPR 18.8-19.0 -> 18.65-18.72 us, TPR 19.6-19.9 -> 18.8-18.9 us)
+// and the persistent uneven and multi-workgroup kernels' much dearer
+// (0.382 -> 0.452 ms, 0.098 -> 0.122 ms), so only the batched kernel asks.
+template <int E, bool NT = false>
 __device__ __forceinline__ void store_e(float* __restrict__ p, const float (&v)[E]) {
-    if constexpr (E % 4 == 0) {
+    if constexpr (E % 4 == 0 && NT) {
+        typedef float v4f __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int i = 0; i < E / 4; i++) {
+            const v4f t = {v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
+            __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p) + i);
+        }
+    } else if constexpr (E % 4 == 0) {
 #pragma unroll
         for (int i = 0; i < E / 4; i++) {
             reinterpret_cast<float4*>(p)[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
