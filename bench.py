@@ -173,10 +173,11 @@ def batch_stream_leg(args, sets, shape):
     trace's per-launch durations are longer than the time per batch here).
     Eight buffer sets (671 MB, well past the 256 MB last-level cache), K
     launches captured in one graph with a fork and a join, HIP events
-    around the replay."""
+    around the replay.  The launches store x through the caches
+    (tprh_solve_desc.x_cached = 1): non-temporal stores, the default for
+    one batch at a time, lose HBM efficiency when launches overlap."""
     import torch
 
-    import tpr_amd
     from tpr_amd import batched as B
 
     nsets = 2 * ROT
@@ -189,10 +190,10 @@ def batch_stream_leg(args, sets, shape):
 
     def step(i):
         t = pool[i % nsets]
-        tpr_amd.solve_batched(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith,
-                              shape=shape)
+        B.solve_ex(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith, shape=shape,
+                   x_cached=True)
 
-    out = {"buffer_sets": nsets}
+    out = {"buffer_sets": nsets, "x_stores": "cached"}
     for ns in (2, 3):
         side = [torch.cuda.Stream() for _ in range(ns - 1)]
         for i in range(nsets):

@@ -186,6 +186,10 @@ typedef struct tprh_solve_desc {
                                than the device holds); 3 = pipelined (one persistent launch of
                                rows, reduced-system and stage-3 tasks; slower than two passes
                                on MI355X).  Exact answers are the same bits in every form. */
+    int x_cached;           /* systems within one workgroup: 0 = x written with non-temporal
+                               stores (fastest for one batch at a time); 1 = x written through
+                               the caches (faster when launches overlap on several streams).
+                               Same answers either way. */
 } tprh_solve_desc;
 
 /* Workspace bytes tprh_solve_ex needs for desc (0 when none); (size_t)-1 if

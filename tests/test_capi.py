@@ -128,6 +128,12 @@ def test_workspace_sizing_without_device():
         d.split_form = form
         assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
     d.split_form = 0
+    for flag in (-1, 2):                                          # x_cached is a flag
+        d.x_cached = flag
+        assert L.tprh_workspace_bytes(ctypes.byref(d)) == ctypes.c_size_t(-1).value
+    d.x_cached = 1
+    assert L.tprh_workspace_bytes(ctypes.byref(d)) == 4 * 3 * 9 * 256 + 128 * (3 + 1) + 256
+    d.x_cached = 0
     # a large system without a workspace is refused before any launch
     d.refine, d.coef, d.coef_shared = 0, 0, 0
     d.a = d.b = d.c = d.d = d.x = 16

@@ -123,6 +123,30 @@ def test_alternative_shapes_bit_identical(s, E, L, method):
     assert bits_equal(x, O.solve_batch(method, a, b, c, d, n, s))
 
 
+# systems of one wavefront, of up to 512 lanes, and of 1024 lanes
+@pytest.mark.parametrize("n,s", [(1024, 256), (4096, 256), (16384, 256)])
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_cached_x_stores_same_answers(method, n, s):
+    """x_cached = 1 (x stored through the caches) and the default
+    non-temporal stores write the same bits: the oracle's in exact mode, and
+    each other's in fast mode."""
+    nsys = 21
+    say("x_cached", method, n, s)
+    a, b, c, d, _ = O.gen_batch(17, 0, nsys, n)
+    da, db, dc, dd = to_dev(a, b, c, d)
+    ref = O.solve_batch(method, a, b, c, d, n, s)
+    for arith in ("exact", "fast"):
+        xs = []
+        for cached in (False, True):
+            dx = torch.full_like(da, float("nan"))
+            B.solve_ex(da, db, dc, dd, dx, n, s, method=method, arith=arith, x_cached=cached)
+            torch.cuda.synchronize()
+            xs.append(dx.cpu().numpy())
+        assert bits_equal(xs[0], xs[1])
+        if arith == "exact":
+            assert bits_equal(xs[1], ref)
+
+
 @pytest.mark.parametrize("m", [1, 4, 16])
 @pytest.mark.parametrize("s", [256, 512])
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])

@@ -61,7 +61,7 @@ class SolveDesc(ctypes.Structure):
         ("d", ctypes.c_void_p), ("x", ctypes.c_void_p),
         ("refine", ctypes.c_int), ("E", ctypes.c_int), ("L", ctypes.c_int), ("chunk", ctypes.c_int),
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
-        ("coef_shared", ctypes.c_int), ("split_form", ctypes.c_int),
+        ("coef_shared", ctypes.c_int), ("split_form", ctypes.c_int), ("x_cached", ctypes.c_int),
     ]
 
 

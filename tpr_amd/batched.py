@@ -64,10 +64,11 @@ def _dptr_any(t, name, numel, dtypes):
 
 
 def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
-              shared=False, split_form=0):
+              shared=False, split_form=0, x_cached=False):
     """tprh_solve_desc for torch tensors; a, b, c float32 or float16 (same
     dtype).  shared: a, b, c hold one matrix (n values) for every right-hand
-    side in d."""
+    side in d.  x_cached: store x through the caches (for launches kept in
+    flight on several streams) instead of non-temporally."""
     import torch
     nsys = d.numel() // n
     tot = nsys * n
@@ -89,6 +90,7 @@ def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sh
     desc.chunk = int(chunk)
     desc.coef_shared = 1 if shared else 0
     desc.split_form = int(split_form)
+    desc.x_cached = 1 if x_cached else 0
     return desc
 
 
@@ -100,16 +102,17 @@ def workspace_bytes(desc):
 
 
 def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, shape=None, chunk=0,
-             workspace=None, stream=None, shared=False, split_form=0):
+             workspace=None, stream=None, shared=False, split_form=0, x_cached=False):
     """The general entry point (tprh_solve_ex): large systems (through a
     workspace, allocated here from torch if not given), fp16 coefficient
     storage (a, b, c as float16 tensors), one refinement step (refine=True:
     x += A^-1 (d - A x) with an FP64 residual), many right-hand sides of one
     matrix (shared=True: a, b, c of n values, d and x of nsys * n); split_form
     0 / 1 / 2 / 3: the library's choice / two passes / one pass / pipelined
-    for systems above one workgroup."""
+    for systems above one workgroup; x_cached=True for launches that overlap
+    on several streams (x stored through the caches, same answers)."""
     import torch
-    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared, split_form)
+    desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared, split_form, x_cached)
     need = workspace_bytes(desc)
     own = None
     if need:

@@ -28,7 +28,7 @@ template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B, typename CT
 __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
                                                        const CT* __restrict__ c, const float* __restrict__ d,
                                                        float* __restrict__ x, int n, int m, int nsys,
-                                                       int lg_lanes, int shared) {
+                                                       int lg_lanes, int shared, int x_cached) {
     extern __shared__ float lds[];
     const int sl = threadIdx.x >> lg_lanes;             // system slot in the workgroup
     const int t = threadIdx.x & ((1 << lg_lanes) - 1);  // lane inside the system
@@ -61,7 +61,15 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
 #pragma unroll
         for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
     }
-    if (valid) store_e<E, true>(x + off, X);  // non-temporal: x is not read again here
+    // x is not read again by this launch: non-temporal stores shorten the
+    // launch's tail when batches run one at a time (config 1: 18.4 against
+    // 19.1 us), but cost HBM efficiency when launches overlap on several
+    // streams (16-17 against 20-21 us per batch), where the caller asks for
+    // cached stores
+    if (valid) {
+        if (x_cached) store_e<E>(x + off, X);
+        else store_e<E, true>(x + off, X);
+    }
 }
 
 namespace {
@@ -73,10 +81,12 @@ void launch_b(const BatchArgs& p, int grid, int block, size_t shmem, int lg, hip
     const CT* c = static_cast<const CT*>(p.c);
     if (p.b != nullptr) {
         hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, true, CT, REFINE>), dim3(grid), dim3(block),
-                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg, p.shared_coef ? 1 : 0);
+                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg, p.shared_coef ? 1 : 0,
+                           p.x_cached ? 1 : 0);
     } else {
         hipLaunchKernelGGL((batched_kernel<METHOD, E, L, MODE, BLOCK, false, CT, REFINE>), dim3(grid), dim3(block),
-                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg, p.shared_coef ? 1 : 0);
+                           shmem, stream, a, b, c, p.d, p.x, p.n, p.m, p.nsys, lg, p.shared_coef ? 1 : 0,
+                           p.x_cached ? 1 : 0);
     }
 }
 

@@ -150,6 +150,7 @@ tprh_status plan_desc(const tprh_solve_desc& d, Plan* pl) {
     if (d.refine != 0 && d.refine != 1) return fail(TPRH_ERROR_INVALID_VALUE, "refine must be 0 or 1");
     if (d.coef_shared != 0 && d.coef_shared != 1) return fail(TPRH_ERROR_INVALID_VALUE, "coef_shared must be 0 or 1");
     if (d.split_form < 0 || d.split_form > 3) return fail(TPRH_ERROR_INVALID_VALUE, "split_form must be 0 .. 3");
+    if (d.x_cached != 0 && d.x_cached != 1) return fail(TPRH_ERROR_INVALID_VALUE, "x_cached must be 0 or 1");
     if (d.refine && d.coef != TPRH_STORE_F32)
         return fail(TPRH_ERROR_INVALID_VALUE, "refinement needs fp32 coefficients for the residual");
     const bool explicit_shape = d.E != 0 || d.L != 0;
@@ -182,6 +183,7 @@ tprh_status solve_desc(const tprh_solve_desc& d, hipStream_t stream) {
         p.f16 = d.coef == TPRH_STORE_F16;
         p.refine = d.refine != 0;
         p.shared_coef = d.coef_shared != 0;
+        p.x_cached = d.x_cached != 0;
         e = tprh::launch_batched(method, mode, p, pl.E, pl.L, stream);
     } else {
         if (!d.workspace || d.workspace_bytes < pl.ws)

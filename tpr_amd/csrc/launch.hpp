@@ -73,6 +73,7 @@ struct BatchArgs {
     bool f16 = false;     // a, b, c stored as fp16
     bool refine = false;  // x += A^-1 (d - A x), residual in FP64
     bool shared_coef = false;  // every system uses the one matrix at a, b, c
+    bool x_cached = false;     // store x through the caches (false: non-temporal stores)
 };
 
 // method 0 = PTPR, 1 = TPR; mode 0 = fast, 1 = exact.

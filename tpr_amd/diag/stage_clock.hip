@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const flo
     fast16_stage3<E>(st, xprev, xe, se, l, X);
     keep(X);
     t[4] = stamp();
-    if (valid) store_e<E>(x + off, X);
+    if (valid) store_e<E, true>(x + off, X);
     __builtin_amdgcn_s_waitcnt(0);
     t[5] = stamp();
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
