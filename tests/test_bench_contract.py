@@ -49,4 +49,4 @@ def test_json_line(args):
         assert d["accuracy"]["rel_l2_vs_cpu_max"] <= 1e-5
         # the reference's analytic count for PTPR, N = 1024, S = 256
         assert d["analytic_flops"]["per_system"] == 118963
-        assert set(d["batch_stream"]) == {"buffer_sets", "2_streams", "3_streams"}
+        assert set(d["batch_stream"]) == {"buffer_sets", "x_stores", "2_streams", "3_streams"}
