@@ -173,9 +173,8 @@ def batch_stream_leg(args, sets, shape):
     trace's per-launch durations are longer than the time per batch here).
     Eight buffer sets (671 MB, well past the 256 MB last-level cache), K
     launches captured in one graph with a fork and a join, HIP events
-    around the replay.  The launches store x through the caches
-    (tprh_solve_desc.x_cached = 1): non-temporal stores, the default for
-    one batch at a time, lose HBM efficiency when launches overlap."""
+    around the replay, with the library's default (non-temporal) stores of
+    x."""
     import torch
 
     from tpr_amd import batched as B
@@ -190,10 +189,9 @@ def batch_stream_leg(args, sets, shape):
 
     def step(i):
         t = pool[i % nsets]
-        B.solve_ex(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith, shape=shape,
-                   x_cached=True)
+        B.solve_ex(t["a"], t["b"], t["c"], t["d"], t["x"], N, S, method=args.method, arith=args.arith, shape=shape)
 
-    out = {"buffer_sets": nsets, "x_stores": "cached"}
+    out = {"buffer_sets": nsets, "x_stores": "non-temporal"}
     for ns in (2, 3):
         side = [torch.cuda.Stream() for _ in range(ns - 1)]
         for i in range(nsets):

@@ -187,8 +187,9 @@ typedef struct tprh_solve_desc {
                                rows, reduced-system and stage-3 tasks; slower than two passes
                                on MI355X).  Exact answers are the same bits in every form. */
     int x_cached;           /* systems within one workgroup: 0 = x written with non-temporal
-                               stores (fastest for one batch at a time); 1 = x written through
-                               the caches (faster when launches overlap on several streams).
+                               stores (the default; measured fastest for one batch at a time and
+                               for launches overlapping on several streams); 1 = x written
+                               through the caches, for a caller whose next kernel reads x.
                                Same answers either way. */
 } tprh_solve_desc;
 

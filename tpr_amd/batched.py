@@ -67,8 +67,8 @@ def make_desc(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sh
               shared=False, split_form=0, x_cached=False):
     """tprh_solve_desc for torch tensors; a, b, c float32 or float16 (same
     dtype).  shared: a, b, c hold one matrix (n values) for every right-hand
-    side in d.  x_cached: store x through the caches (for launches kept in
-    flight on several streams) instead of non-temporally."""
+    side in d.  x_cached: store x through the caches (for a caller whose
+    next kernel reads x) instead of non-temporally."""
     import torch
     nsys = d.numel() // n
     tot = nsys * n
@@ -109,8 +109,8 @@ def solve_ex(a, b, c, d, x, n, s, method="ptpr", arith="fast", refine=False, sha
     x += A^-1 (d - A x) with an FP64 residual), many right-hand sides of one
     matrix (shared=True: a, b, c of n values, d and x of nsys * n); split_form
     0 / 1 / 2 / 3: the library's choice / two passes / one pass / pipelined
-    for systems above one workgroup; x_cached=True for launches that overlap
-    on several streams (x stored through the caches, same answers)."""
+    for systems above one workgroup; x_cached=True stores x through the
+    caches instead of non-temporally (same answers)."""
     import torch
     desc = make_desc(a, b, c, d, x, n, s, method, arith, refine, shape, chunk, shared, split_form, x_cached)
     need = workspace_bytes(desc)

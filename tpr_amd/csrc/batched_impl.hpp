@@ -15,6 +15,7 @@
 
 #include "launch.hpp"
 #include "slice_solvers.hpp"
+#include "staged_io.hpp"
 
 namespace tprh {
 
@@ -26,91 +27,6 @@ namespace tprh {
 // fits in one wavefront (m * L <= 64) and its reduced solve synchronises
 // per wavefront, so the four wavefronts of a workgroup run independently.
 constexpr int kSmallBlock = 256;
-
-// x of the grid is written linearly: lane g (blockIdx.x * blockDim.x +
-// threadIdx.x) owns x[g E .. g E + E), so a wavefront's 64 lanes own one
-// contiguous run of 64 E floats.  Stored lane by lane, every 16-byte store
-// instruction writes E / 4 separate pieces of each 4 E-byte lane range, and
-// the non-temporal stores reach memory as partial lines (1.5 times the bytes
-// of x written, E = 16).  kStageX: such shapes pass x through LDS so that
-// each store instruction writes 1 KB of consecutive bytes.
-template <int E>
-constexpr bool kStageX = E == 8 || E == 16;
-
-// Words of LDS per workgroup for the x staging (after the systems' rows,
-// whose size is rounded up to 16 bytes).
-template <int E>
-__host__ __device__ constexpr int stage_words(int block) { return block * E; }
-__host__ __device__ constexpr int stage_base(int spb, int m) { return (spb * lds_words_per_system(m) + 3) & ~3; }
-
-template <int E, bool NT>
-__device__ __forceinline__ void store_x_staged(float* __restrict__ x, const float (&X)[E], float* stage,
-                                               size_t wave_off, size_t total, int lane) {
-    typedef float v4f __attribute__((ext_vector_type(4)));
-    v4f* s4 = reinterpret_cast<v4f*>(stage);
-#pragma unroll
-    for (int i = 0; i < E / 4; i++) s4[lane * (E / 4) + i] = v4f{X[4 * i], X[4 * i + 1], X[4 * i + 2], X[4 * i + 3]};
-    group_sync<true>();
-#pragma unroll
-    for (int k = 0; k < E / 4; k++) {
-        const int q = k * 64 + lane;  // 16-byte piece of the wavefront's run
-        const size_t gi = wave_off + 4 * (size_t)q;
-        const v4f v = s4[q];
-        if (gi < total) {
-            if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<v4f*>(x + gi));
-            else *reinterpret_cast<v4f*>(x + gi) = v;
-        }
-    }
-}
-
-// The inputs in the same linear layout (lane g owns a, b, c, d [g E, g E + E)),
-// loaded as 1 KB runs per instruction and handed to their lanes through LDS
-// (stage: 64 E floats of the wavefront).  Pieces past the batch load the
-// batch's last piece instead: the lanes they reach never store.
-template <int E, bool HAS_B>
-__device__ __forceinline__ void load_raw_staged(const float* __restrict__ a, const float* __restrict__ b,
-                                                const float* __restrict__ c, const float* __restrict__ d,
-                                                size_t wave_off, size_t total, float* stage, int lane,
-                                                RawPart<E>& p) {
-    typedef float v4f __attribute__((ext_vector_type(4)));
-    constexpr int K = E / 4;
-    size_t gi[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const size_t g = wave_off + 4 * (size_t)(k * 64 + lane);
-        gi[k] = g < total ? g : total - 4;
-    }
-    v4f la[K], lb[K], lc[K], ld[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) la[k] = *reinterpret_cast<const v4f*>(a + gi[k]);
-#pragma unroll
-    for (int k = 0; k < K; k++) lc[k] = *reinterpret_cast<const v4f*>(c + gi[k]);
-#pragma unroll
-    for (int k = 0; k < K; k++) ld[k] = *reinterpret_cast<const v4f*>(d + gi[k]);
-    if constexpr (HAS_B) {
-#pragma unroll
-        for (int k = 0; k < K; k++) lb[k] = *reinterpret_cast<const v4f*>(b + gi[k]);
-    }
-    v4f* s4 = reinterpret_cast<v4f*>(stage);
-    auto turn = [&](const v4f(&l)[K], float(&out)[E]) {
-#pragma unroll
-        for (int k = 0; k < K; k++) s4[k * 64 + lane] = l[k];
-        group_sync<true>();
-#pragma unroll
-        for (int i = 0; i < K; i++) {
-            const v4f v = s4[lane * K + i];
-            out[4 * i] = v.x;
-            out[4 * i + 1] = v.y;
-            out[4 * i + 2] = v.z;
-            out[4 * i + 3] = v.w;
-        }
-        group_sync<true>();
-    };
-    turn(la, p.a);
-    turn(lc, p.c);
-    turn(ld, p.d);
-    if constexpr (HAS_B) turn(lb, p.b);
-}
 
 template <int METHOD, int E, int L, int MODE, int BLOCK, bool HAS_B, typename CT, bool REFINE>
 __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a, const CT* __restrict__ b,
@@ -128,7 +44,7 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
     const size_t coff = shared ? (size_t)t * E : off;  // shared: one matrix for every right-hand side
     // the staged layout: this wavefront's run of 64 E floats and its LDS
     constexpr bool STAGE = kStageX<E>;
-    float* stage = lds + stage_base(blockDim.x >> lg_lanes, m) + (threadIdx.x & ~63) * E;
+    float* stage = lds + stage_base(blockDim.x >> lg_lanes, m) + (threadIdx.x >> 6) * stage_wave_words<E>();
     const size_t wave_off = ((size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) * E;
     const size_t total = (size_t)nsys * (size_t)n;
     RawPart<E> raw;
@@ -159,11 +75,10 @@ __global__ __launch_bounds__(BLOCK) void batched_kernel(const CT* __restrict__ a
 #pragma unroll
         for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
     }
-    // x is not read again by this launch: non-temporal stores shorten the
-    // launch's tail when batches run one at a time (config 1: 18.4 against
-    // 19.1 us), but cost HBM efficiency when launches overlap on several
-    // streams (16-17 against 20-21 us per batch), where the caller asks for
-    // cached stores
+    // x is not read again by this launch: non-temporal stores by default
+    // (4096 x N=1024 with staged stores: 16.5 against 17.2 us on one stream,
+    // 13.5 against 13.6 us per batch on three); x_cached keeps x in the
+    // caches for a caller whose next kernel reads it
     if constexpr (STAGE) {
         // every lane takes part (the pieces past the batch are not stored)
         if (x_cached) store_x_staged<E, false>(x, X, stage, wave_off, total, threadIdx.x & 63);
@@ -184,7 +99,7 @@ hipError_t launch_b(const BatchArgs& p, int grid, int block, size_t shmem, int l
     auto* kern = p.b != nullptr ? batched_kernel<METHOD, E, L, MODE, BLOCK, true, CT, REFINE>
                                 : batched_kernel<METHOD, E, L, MODE, BLOCK, false, CT, REFINE>;
     if (shmem > 65536) {
-        // 1024-lane systems of 16 equations per lane stage 64 KB of x
+        // 1024-lane systems of 16 equations per lane stage 68 KB
         static std::atomic<unsigned long long> set_b{0}, set_u{0};
         hipError_t e = ensure_lds_limit(reinterpret_cast<const void*>(kern), (int)shmem, p.b != nullptr ? set_b : set_u);
         if (e != hipSuccess) return e;
