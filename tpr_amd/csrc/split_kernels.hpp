@@ -5,8 +5,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "launch_split.hpp"
 #include "slice_solvers.hpp"
+#include "staged_io.hpp"
 
 namespace tprh {
 
@@ -54,8 +57,29 @@ __device__ __forceinline__ void slice_index(long long gs, int lg_m, int l, long 
 template <int METHOD, int E, int L, int MODE>
 constexpr bool kSplitTwoLevel = METHOD == 0 && MODE == kFast && L == 16 && E >= 2;
 
+// K1 and K3 load (and K3 stores) their slices as 1 KB runs per instruction
+// through LDS (staged_io.hpp) where the shape stages and the coefficients are
+// fp32: a wavefront holds 64 / L whole slices of one system (m L > 1024), one
+// contiguous run starting at `run`.  256-lane workgroups: four wavefronts.
+template <int E, typename CT>
+constexpr bool kSplitStaged = kStageX<E> && std::is_same<CT, float>::value;
+template <int E, typename CT>
+constexpr int kSplitStageWords = kSplitStaged<E, CT> ? 4 * stage_wave_words<E>() : 4;
+
+template <int E, bool HAS_B, typename CT, typename Loc>
+__device__ __forceinline__ void load_slices(const Loc& loc, long long q, size_t lo, size_t run, float* stage,
+                                            RawPart<E>& raw) {
+    if constexpr (kSplitStaged<E, CT>) {
+        load_raw_staged<E, HAS_B>(loc.A(q), HAS_B ? loc.B(q) : nullptr, loc.C(q), loc.D(q), run, (size_t)loc.n,
+                                  stage, threadIdx.x & 63, raw);
+    } else {
+        load_raw_t<E, CT>(loc.A(q), HAS_B ? loc.B(q) : nullptr, loc.C(q), loc.D(q), lo, lo, raw);
+    }
+}
+
 template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
 __global__ __launch_bounds__(256) void rows_kernel(Loc loc, int lg_m, long long nslices) {
+    __shared__ __align__(16) float stage_lds[kSplitStageWords<E, CT>];
     const int l = threadIdx.x % L;
     const long long g = (long long)blockIdx.x * (blockDim.x / L) + threadIdx.x / L;  // slice over the round
     const bool valid = g < nslices;
@@ -63,8 +87,11 @@ __global__ __launch_bounds__(256) void rows_kernel(Loc loc, int lg_m, long long 
     int j;
     size_t lo;
     slice_index<E, L>(valid ? g : 0, lg_m, l, q, j, lo);
+    // valid is uniform across a wavefront (nslices is a multiple of m)
+    const size_t run = valid ? lo - (size_t)(threadIdx.x & 63) * E : 0;
+    float* stage = kSplitStaged<E, CT> ? stage_lds + (threadIdx.x >> 6) * stage_wave_words<E>() : stage_lds;
     RawPart<E> raw;
-    load_raw_t<E, CT>(loc.A(q), HAS_B ? loc.B(q) : nullptr, loc.C(q), loc.D(q), lo, lo, raw);
+    load_slices<E, HAS_B, CT>(loc, q, lo, run, stage, raw);
     float* row = loc.Rows(q) + (size_t)j * kRowWords;
     if constexpr (REFINE) {
         float xo[E], xl, xr;
@@ -189,6 +216,7 @@ __global__ __launch_bounds__(1024) void reduced_kernel(Loc loc) {
 
 template <int METHOD, int E, int L, int MODE, bool HAS_B, typename CT, bool REFINE, typename Loc>
 __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long long nslices) {
+    __shared__ __align__(16) float stage_lds[kSplitStageWords<E, CT>];
     const int l = threadIdx.x % L;
     // workgroups take the slices in the reverse of K1's order: the slices K1
     // read last are the likeliest still in the device's last-level cache
@@ -199,8 +227,10 @@ __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long lon
     int j;
     size_t lo;
     slice_index<E, L>(valid ? g : 0, lg_m, l, q, j, lo);
+    const size_t run = valid ? lo - (size_t)(threadIdx.x & 63) * E : 0;
+    float* stage = kSplitStaged<E, CT> ? stage_lds + (threadIdx.x >> 6) * stage_wave_words<E>() : stage_lds;
     RawPart<E> raw;
-    load_raw_t<E, CT>(loc.A(q), HAS_B ? loc.B(q) : nullptr, loc.C(q), loc.D(q), lo, lo, raw);
+    load_slices<E, HAS_B, CT>(loc, q, lo, run, stage, raw);
     float* x = loc.X(q);
     const float* rows = loc.Rows(q);
     float xo[E];
@@ -252,7 +282,11 @@ __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long lon
 #pragma unroll
         for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
     }
-    if (valid) store_e<E>(x + lo, X);
+    if constexpr (kSplitStaged<E, CT>) {
+        if (valid) store_x_staged<E, false>(x, X, stage, run, (size_t)loc.n, threadIdx.x & 63);
+    } else if (valid) {
+        store_e<E>(x + lo, X);
+    }
 }
 
 // ---------------------------------------------------------------------------
