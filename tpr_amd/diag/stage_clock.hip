@@ -1,8 +1,9 @@
 // stage_clock.hip -- a diagnostic build (libtpr_diag.so, not part of the
 // solver library): the batched fast-PTPR solve of one-wavefront systems on
 // 16-lane slices (the shape of the 4096 x N=1024, S=256 batch), assembled
-// from the same device functions as batched_kernel -- load_raw_t,
-// fast16_stage1, reduced_solve_wave, fast16_stage3, store_e -- with the GPU's
+// from the same device functions as batched_kernel -- load_raw_staged
+// (load_raw_t for E < 8), fast16_stage1, reduced_solve_wave, fast16_stage3,
+// store_x_staged (store_e for E < 8) -- with the GPU's
 // constant 100 MHz clock (s_memrealtime) read between them.  Each wave
 // records six stamps: start, loads complete, stage 1 done, reduced system
 // done, stage 3 done, stores complete.  The waits that make "loads complete"
@@ -14,6 +15,7 @@
 
 #include "../csrc/slice_solvers.hpp"
 #include "../csrc/split_kernels.hpp"
+#include "../csrc/staged_io.hpp"
 
 namespace tprh {
 
@@ -54,8 +56,13 @@ __global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const flo
     const bool valid = sys < nsys;
     const size_t x0 = valid ? (size_t)sys * (size_t)n : 0;
     const size_t off = x0 + (size_t)tt * E;
+    // the wavefront's run of 64 E floats and its LDS, as batched_kernel's
+    float* stage = lds + stage_base(blockDim.x >> lg_lanes, m) + (threadIdx.x >> 6) * stage_wave_words<E>();
+    const size_t wave_off = ((size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) * E;
+    const size_t total = (size_t)nsys * (size_t)n;
     RawPart<E> raw;
-    load_raw_t<E, float>(a, b, c, d, off, off, raw);
+    if constexpr (kStageX<E>) load_raw_staged<E, true>(a, b, c, d, wave_off, total, stage, threadIdx.x & 63, raw);
+    else load_raw_t<E, float>(a, b, c, d, off, off, raw);
     __builtin_amdgcn_s_waitcnt(0);
     t[1] = stamp();
     const int l = tt & 15;
@@ -76,7 +83,8 @@ __global__ __launch_bounds__(kClockBlock) void ptpr_stage_clock_kernel(const flo
     fast16_stage3<E>(st, xprev, xe, se, l, X);
     keep(X);
     t[4] = stamp();
-    if (valid) store_e<E, true>(x + off, X);
+    if constexpr (kStageX<E>) store_x_staged<E, true>(x, X, stage, wave_off, total, threadIdx.x & 63);
+    else if (valid) store_e<E, true>(x + off, X);
     __builtin_amdgcn_s_waitcnt(0);
     t[5] = stamp();
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -131,7 +139,10 @@ extern "C" int tprh_diag_ptpr_stage_clock(const float* a, const float* b, const 
     const int lg = __builtin_ctz((unsigned)T);
     const int spb = kClockBlock / T;
     const int grid = (nsys + spb - 1) / spb;
-    const size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(m);
+    size_t shmem = sizeof(float) * (size_t)spb * lds_words_per_system(m);
+    if (E == 8 || E == 16)  // the staging area, as batched_kernel's
+        shmem = sizeof(float) * ((size_t)stage_base(spb, m) +
+                                 (size_t)(E == 8 ? stage_words<8>(kClockBlock) : stage_words<16>(kClockBlock)));
     hipStream_t st = (hipStream_t)stream;
     switch (E) {
         case 2: hipLaunchKernelGGL(ptpr_stage_clock_kernel<2>, dim3(grid), dim3(kClockBlock), shmem, st, a, b, c, d, x, n, m, nsys, lg, stamps); break;
