@@ -8,7 +8,7 @@
 // stores then reach memory as partial lines (WRITE_SIZE 1.5 times the bytes
 // of x on 4096 x N=1024).  Passed through 68 E floats of LDS per wavefront,
 // every instruction covers 1 KB of consecutive bytes instead, whole lines:
-// 4096 x N=1024 batched PTPR 18.6 -> 15.1-15.8 us.
+// 4096 x N=1024 batched PTPR 18.6 -> 14.3-14.8 us.
 #pragma once
 #include <hip/hip_runtime.h>
 
