@@ -123,6 +123,31 @@ def test_alternative_shapes_bit_identical(s, E, L, method):
     assert bits_equal(x, O.solve_batch(method, a, b, c, d, n, s))
 
 
+# Loads and stores staged through LDS as 1 KB runs per wavefront (E = 8 and
+# 16): several systems per wavefront, one system per wavefront, systems of
+# 128 to 1024 lanes (the last above 64 KB of LDS), with a last workgroup or
+# wavefront only partly filled.
+STAGED_CASES = [(256, 256, 16, 16, 37), (1024, 256, 16, 16, 37), (512, 128, 8, 16, 37), (4096, 256, 16, 16, 3),
+                (1024, 256, 8, 32, 5), (16384, 256, 16, 16, 2), (8192, 512, 8, 64, 3)]
+
+
+@pytest.mark.parametrize("n,s,E,L,nsys", STAGED_CASES)
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
+def test_staged_io_exact_and_in_bounds(method, n, s, E, L, nsys):
+    """Exact answers bit for bit, and nothing written past the batch's x
+    (a guard region behind it keeps its values)."""
+    say("staged", method, n, s, E, L, nsys)
+    a, b, c, d, _ = O.gen_batch(23, 5, nsys, n)
+    da, db, dc, dd = to_dev(a, b, c, d)
+    guard = 4096
+    xfull = torch.full((nsys * n + guard,), 7.25, dtype=torch.float32, device="cuda")
+    B.solve_batched(da, db, dc, dd, xfull[:nsys * n], n, s, method=method, arith="exact", shape=(E, L))
+    torch.cuda.synchronize()
+    xf = xfull.cpu().numpy()
+    assert bits_equal(xf[:nsys * n], O.solve_batch(method, a, b, c, d, n, s))
+    assert np.all(xf[nsys * n:] == 7.25)
+
+
 # systems of one wavefront, of up to 512 lanes, and of 1024 lanes
 @pytest.mark.parametrize("n,s", [(1024, 256), (4096, 256), (16384, 256)])
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
