@@ -63,6 +63,23 @@ def test_split_exact_bit_identical(method, n, s, nsys, chunk):
 
 
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
+@pytest.mark.parametrize("n,s,nsys,chunk", [(65536, 256, 3, 2), (32768, 512, 2, 0), (16384, 128, 3, 0)])
+def test_split_staged_stores_in_bounds(method, n, s, nsys, chunk):
+    """The two-pass form's staged loads and stores (E = 8 and 16): exact
+    answers bit for bit and nothing written past the batch's x."""
+    say("split staged", method, n, s, nsys, chunk)
+    a, b, c, d, _ = O.gen_batch(37, 0, nsys, n, O.GEN_WEAK)
+    guard = 4096
+    xfull = torch.full((nsys * n + guard,), -3.5, dtype=torch.float32, device="cuda")
+    B.solve_ex(dev(a), dev(b), dev(c), dev(d), xfull[:nsys * n], n, s, method=method, arith="exact", chunk=chunk,
+               split_form=1)
+    torch.cuda.synchronize()
+    xf = xfull.cpu().numpy()
+    assert bits_equal(xf[:nsys * n], O.solve_batch(method, a, b, c, d, n, s))
+    assert np.all(xf[nsys * n:] == -3.5)
+
+
+@pytest.mark.parametrize("method", ["ptpr", "tpr"])
 @pytest.mark.parametrize("n,s", [(1 << 22, 2048), (1 << 19, 256)])
 def test_split_maximum_sizes(method, n, s):
     """The largest systems the engine takes: m = n / s = 2048 slices (the
