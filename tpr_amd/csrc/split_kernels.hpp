@@ -61,6 +61,8 @@ constexpr bool kSplitTwoLevel = METHOD == 0 && MODE == kFast && L == 16 && E >= 
 // through LDS (staged_io.hpp) where the shape stages and the coefficients are
 // fp32: a wavefront holds 64 / L whole slices of one system (m L > 1024), one
 // contiguous run starting at `run`.  256-lane workgroups: four wavefronts.
+// K3 stores x non-temporally (256 x N=65536: 0.0951-0.0955 against
+// 0.0965-0.0968 ms with cached stores).
 template <int E, typename CT>
 constexpr bool kSplitStaged = kStageX<E> && std::is_same<CT, float>::value;
 template <int E, typename CT>
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256) void stage3_kernel(Loc loc, int lg_m, long lon
         for (int i = 0; i < E; i++) X[i] = xo[i] + X[i];
     }
     if constexpr (kSplitStaged<E, CT>) {
-        if (valid) store_x_staged<E, false>(x, X, stage, run, (size_t)loc.n, threadIdx.x & 63);
+        if (valid) store_x_staged<E, true>(x, X, stage, run, (size_t)loc.n, threadIdx.x & 63);
     } else if (valid) {
         store_e<E>(x + lo, X);
     }
