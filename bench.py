@@ -1126,9 +1126,9 @@ def main():
         out["cpu_baseline"] = base
         out["accuracy"] = acc
     if side:
-        # the other configurations of the metric, from short runs: the
-        # serving loop's latency (config 4), the uneven batch (config 2) and
-        # the precision variants (config 3)
+        # the rest of the metric, from short runs: the serving loop's
+        # latency, the 20000-system uneven batch and the precision variants
+        # of 256 x N=65536
         sv = serve_summary(args, poisson, closed, serve_per_system(args, 100))
         out["latency_ms"] = sv["latency_ms"]
         out["serve"] = sv

@@ -1,17 +1,17 @@
-"""Parity at the benchmark configurations' full sizes, not only at reduced
+"""Parity at the benchmark workloads' full sizes, not only at reduced
 ones (the reference's own tests assert only the last case of their loops,
 test/examples.cpp:76-102; these check every class and every system):
 
-* config 2 -- 20000 systems, N = 2^U{8..14}, the committed autotuned
+* the uneven batch -- 20000 systems, N = 2^U{8..14}, the committed autotuned
   schedule, one persistent launch: exact arithmetic bit-identical to the
   oracle on a stratified sample of 2100 systems (300 of every size class);
   fast arithmetic within 1e-5 of the generator's truth on all 20000;
-* config 3 -- 256 weakly dominant systems x N=65536 through the
+* the large systems -- 256 weakly dominant systems x N=65536 through the
   multi-workgroup path: exact bit-identical to the oracle on all 256; each
   precision variant's error per system against the host FP64 Thomas solve
   held at the level it reaches (fp32 <= 1e-6, fp32 + 1 refinement step
   <= 5e-8, fp16 coefficients + 1 step <= 1e-7, + 2 steps <= 5e-8);
-* config 4 -- the serving client (tpr_amd/tpr_serve_bench) for 2 s of
+* serving -- the serving client (tpr_amd/tpr_serve_bench) for 2 s of
   Poisson arrivals at 2000 req/s with the 64 GB budget: every request
   completes, and the answers of every 40th request (all its right-hand
   sides, written out by the client) equal the oracle's within 1e-5.
@@ -53,7 +53,7 @@ def per_system_rel(x, ref, n):
 
 
 @pytest.fixture(scope="module")
-def config2():
+def uneven20000():
     sizes = log_uniform_sizes(20000, 8, 14, seed=2020)
     sched = autotune.schedule_of(autotune.load())
     probe = UnevenPlan(sizes, "ptpr", "fast", sched)
@@ -68,12 +68,12 @@ def config2():
 
 
 @pytest.mark.parametrize("method", ["ptpr", "tpr"])
-def test_config2_full_batch(config2, method):
-    sizes, sched, offs, h = config2["sizes"], config2["sched"], config2["offs"], config2["host"]
-    a, b, c, d = config2["dev"]
+def test_uneven_20000_full_batch(uneven20000, method):
+    sizes, sched, offs, h = uneven20000["sizes"], uneven20000["sched"], uneven20000["offs"], uneven20000["host"]
+    a, b, c, d = uneven20000["dev"]
     x = torch.empty_like(a)
     # fast: every system against the truth
-    say("config 2 fast", method)
+    say("uneven 20000 fast", method)
     with UnevenPlan(sizes, method, "fast", sched) as p:
         p.solve(a, b, c, d, x)
         torch.cuda.synchronize()
@@ -83,11 +83,11 @@ def test_config2_full_batch(config2, method):
         ks = np.nonzero(sizes == n)[0]
         idx = np.concatenate([np.arange(offs[k], offs[k + 1]) for k in ks])
         worst = max(worst, per_system_rel(xf[idx], h["xt"][idx], n).max())
-    say("config 2 fast worst rel-L2 vs truth", worst)
+    say("uneven 20000 fast worst rel-L2 vs truth", worst)
     assert worst <= 1e-5
     # exact: 300 systems of every class, bit for bit against the oracle at the
     # class's slice size
-    say("config 2 exact", method)
+    say("uneven 20000 exact", method)
     with UnevenPlan(sizes, method, "exact", sched) as p:
         p.solve(a, b, c, d, x)
         torch.cuda.synchronize()
@@ -105,9 +105,9 @@ def test_config2_full_batch(config2, method):
     assert checked >= 2000
 
 
-def test_config3_full_batch():
+def test_split_256x65536_full_batch():
     nsys, n, s = 256, 65536, 256
-    say("config 3 generate")
+    say("split 256x65536 generate")
     t = B.alloc_batch(nsys, n)
     B.generate("weak", 30303, 0, nsys, n, t["a"], t["b"], t["c"], t["d"], t["xt"])
     torch.cuda.synchronize()
@@ -116,11 +116,11 @@ def test_config3_full_batch():
     errs = {}
     x = t["x"]
     # exact: all 256 systems bit for bit
-    say("config 3 exact")
+    say("split 256x65536 exact")
     B.solve_ex(t["a"], t["b"], t["c"], t["d"], x, n, s, arith="exact")
     torch.cuda.synchronize()
     assert bits_equal(x.cpu().numpy(), O.solve_batch("ptpr", h["a"], h["b"], h["c"], h["d"], n, s))
-    say("config 3 precision variants")
+    say("split 256x65536 precision variants")
     B.solve_ex(t["a"], t["b"], t["c"], t["d"], x, n, s)
     torch.cuda.synchronize()
     errs["fp32"] = per_system_rel(x.cpu().numpy(), xf, n).max()
@@ -143,10 +143,10 @@ def test_config3_full_batch():
     assert errs["fp16+2"] <= 5e-8
 
 
-def test_config4_poisson_run_at_full_budget(tmp_path):
+def test_serving_poisson_run_at_full_budget(tmp_path):
     exe = os.path.join(ROOT, "tpr_amd", "tpr_serve_bench")
     dump = str(tmp_path / "answers.bin")
-    say("config 4 serving run")
+    say("serving serving run")
     p = subprocess.run([exe, "--rate", "2000", "--seconds", "2", "--budget-gb", "64", "--seed", "4044",
                         "--min-batch", "1024", "--max-wait-us", "100", "--dump", dump, "--dump-every", "40"],
                        capture_output=True, text=True, timeout=120)
@@ -169,6 +169,6 @@ def test_config4_poisson_run_at_full_budget(tmp_path):
             ref = O.solve_batch("ptpr", a, b, c, O.gen_rhs(4044, k, j0 + j, n), n, min(n, 256))
             worst = max(worst, O.rel_l2(xs[j], ref))
         checked += 1
-    say("config 4 checked requests", checked, "worst", worst)
+    say("serving checked requests", checked, "worst", worst)
     assert checked >= res["requests"] // 40 - 1
     assert worst <= 1e-5

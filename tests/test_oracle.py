@@ -283,7 +283,7 @@ def test_flops_match_reference_fixture(golden):
 
 def test_flops_fixture_holds_the_documented_count(golden):
     # the count the reference's documented run (tpr_pm 2048 512 ... PTPR)
-    # reports per solve, and config 1's
+    # reports per solve, and that of N=1024, S=256
     assert golden["flops"]["ptpr"]["2048,512"] == 266419
     assert golden["flops"]["ptpr"]["1024,256"] == 118963
 
