@@ -190,7 +190,9 @@ typedef struct tprh_solve_desc {
                                stores (the default; measured fastest for one batch at a time and
                                for launches overlapping on several streams); 1 = x written
                                through the caches, for a caller whose next kernel reads x.
-                               Same answers either way. */
+                               Same answers either way.  Larger systems: the two-pass form
+                               writes x non-temporally, the one-pass and pipelined forms
+                               through the caches, whatever this flag says. */
 } tprh_solve_desc;
 
 /* Workspace bytes tprh_solve_ex needs for desc (0 when none); (size_t)-1 if
